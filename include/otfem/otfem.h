@@ -1,0 +1,170 @@
+/*
+ * otfem — MI355X-native corotated tetrahedral FEM step (C ABI).
+ *
+ * This is the drop-in boundary for the OpenTissue::fem path.  The reference
+ * is a header-only C++ template API with no FFI of its own; the entry points
+ * below are what a binding of that path needs, and each names the reference
+ * function it replaces (paths relative to the OpenTissue tree).  The C++
+ * drop-in header <otfem/fem.hpp> keeps the reference's fem::init /
+ * fem::simulate / fem::update_original_coord / fem::export_K signatures on top
+ * of these calls.
+ *
+ * Conventions
+ *   - Plain pointers and sizes only; every function returns an otfem_status.
+ *   - Host arrays are in the caller's node / tetrahedron order, xyz
+ *     interleaved (node i at [3i, 3i+1, 3i+2]); the library keeps its own
+ *     device layout and permutes at this boundary.
+ *   - One HIP stream per handle.  Calls on one handle must be serialised by
+ *     the caller; distinct handles are independent.
+ *   - No exception crosses the ABI; otfem_last_error() gives a message.
+ */
+#ifndef OTFEM_OTFEM_H
+#define OTFEM_OTFEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  OTFEM_OK = 0,
+  OTFEM_INVALID_ARG = 1,   /* bad pointer, size, index or parameter */
+  OTFEM_INVERTED_TET = 2,  /* a rest volume V <= 0 (reference asserts V>0) */
+  OTFEM_NONFINITE = 3,     /* a CG scalar became NaN/Inf in some body */
+  OTFEM_NOT_CONVERGED = 4, /* informational: a body hit the iteration cap */
+  OTFEM_HIP_ERROR = 5,     /* HIP runtime error or device barrier time-out */
+  OTFEM_NO_DEVICE = 6,     /* no usable gfx950 device */
+  OTFEM_STATE = 7          /* call out of order (e.g. step before init) */
+} otfem_status;
+
+/* Rotation extraction.  GRAM_SCHMIDT is the reference's
+ * (fem_update_orientation.h:208 -> math_matrix3x3.h:535-557); POLAR is the
+ * eigen-based polar decomposition the reference leaves commented out
+ * (fem_update_orientation.h:210-213, math_polar_decomposition.h:49-91). */
+enum { OTFEM_ROT_GRAM_SCHMIDT = 0, OTFEM_ROT_POLAR = 1 };
+
+/* CG stopping rule.  ABS_R2 is the reference's
+ * `iteration >= min && r.r < tol` (fem_conjugate_gradients.h:118); REL_B stops
+ * when sqrt(r.r) <= tol * sqrt(b.b) over free degrees of freedom. */
+enum { OTFEM_TOL_ABS_R2 = 0, OTFEM_TOL_REL_B = 1 };
+
+/* Warm start of CG.  PREVIOUS_VELOCITY is the reference's
+ * (fem_conjugate_gradients.h:48-67: CG starts from m_velocity). */
+enum { OTFEM_WARM_PREVIOUS_VELOCITY = 0, OTFEM_WARM_LEARNED = 1 };
+
+typedef struct {
+  double alpha;          /* mass damping: C = alpha*M + beta*K (ref: 2, fem_simulate.h:104) */
+  double beta;           /* stiffness damping (ref: 0, fem_dynamics_assembly.h:39-41) */
+  int cg_min_iterations; /* ref: 20 (fem_simulate.h:108) */
+  int cg_max_iterations; /* ref: 20 (fem_simulate.h:109) */
+  int tol_mode;          /* OTFEM_TOL_* (ref: ABS_R2) */
+  double tol;            /* ref: 1e-3 (fem_conjugate_gradients.h:44) */
+  double tiny;           /* |p.Ap|, |r.r| clamp (ref: 1e-10, :43); 0 disables */
+  int rotation_mode;     /* OTFEM_ROT_* (ref: GRAM_SCHMIDT) */
+  int warm_start;        /* OTFEM_WARM_* (ref: PREVIOUS_VELOCITY) */
+  int store_rotations;   /* 1: keep per-tet Re for otfem_get_rotations */
+} otfem_params;
+
+typedef struct {
+  int iterations;        /* max CG iterations over bodies this step */
+  int bodies_converged;  /* bodies whose stopping rule fired */
+  int bodies_capped;     /* bodies that reached cg_max_iterations unconverged */
+  int flags;             /* OR of per-body status bits (1: non-finite) */
+  double rr0;            /* body 0: r.r after the warm start */
+  double rr;             /* body 0: final recursive r.r */
+  double bb;             /* body 0: b.b over free dofs */
+  double elapsed_ms;     /* device time of the step (HIP events) */
+} otfem_stats;
+
+typedef struct {
+  int num_nodes;
+  int num_tets;
+  const double *coord;         /* [3N] world coords x           (m_coord) */
+  const double *model_coord;   /* [3N] rest coords x0, or NULL = coord (m_model_coord) */
+  const double *velocity;      /* [3N] or NULL = 0               (m_velocity) */
+  const unsigned char *fixed;  /* [N]  or NULL = none            (m_fixed) */
+  const int32_t *tets;         /* [4T] node indices i, j, k, m   (t4mesh_t4tetrahedron.h:39) */
+  int num_bodies;              /* 0 or 1: one body; >1: CSR of bodies below */
+  const int64_t *body_node_offsets; /* [B+1]: body b owns nodes [o[b], o[b+1]) */
+  const int64_t *body_tet_offsets;  /* [B+1]: body b owns tets  [o[b], o[b+1]) */
+  int target_tile_nodes;       /* owned nodes per device tile, 0 = default */
+} otfem_mesh_desc;
+
+typedef struct otfem_solver *otfem_handle;
+
+/* Library / device. */
+const char *otfem_version(void);
+const char *otfem_status_string(int status);
+/* Reference settings (mode 0) or the relative-residual settings (mode 1:
+ * alpha 0.1, beta 0.001, tol 1e-6, cap 200, tiny off). */
+otfem_status otfem_default_params(otfem_params *params, int mode);
+otfem_status otfem_device_count(int *count);
+
+/* Create a solver for one mesh (or a CSR-of-bodies batch).  Replaces building
+ * a fem::Mesh (fem_mesh.h:21-34, t4mesh_t4mesh.h:178-252) and is the owner of
+ * all solver state the reference keeps in node/tet traits
+ * (fem_node_traits.h:37-52, fem_tetrahedron_traits.h:29-48). */
+otfem_status otfem_create(const otfem_mesh_desc *desc, otfem_handle *out);
+otfem_status otfem_destroy(otfem_handle h);
+const char *otfem_last_error(otfem_handle h);
+
+/* fem::init (fem_init.h:45-65): uniform materials, element stiffness,
+ * lumped mass (fixed nodes -> DBL_MAX), plastic parameters. */
+otfem_status otfem_init(otfem_handle h, double young, double poisson, double density,
+                        double c_yield, double c_creep, double c_max);
+/* The user-written init that fem_init.h:32-35 invites: per-tet E, nu, rho. */
+otfem_status otfem_init_materials(otfem_handle h, const double *young, const double *poisson,
+                                  const double *density, double c_yield, double c_creep,
+                                  double c_max);
+
+/* Node data the reference's caller writes into traits before simulate. */
+otfem_status otfem_set_external_forces(otfem_handle h, const double *f);   /* m_f_external */
+otfem_status otfem_get_external_forces(otfem_handle h, double *f);
+otfem_status otfem_set_state(otfem_handle h, const double *x, const double *v); /* NULL skips */
+otfem_status otfem_get_state(otfem_handle h, double *x, double *v);             /* NULL skips */
+otfem_status otfem_set_fixed(otfem_handle h, const unsigned char *fixed);       /* m_fixed */
+otfem_status otfem_get_mass(otfem_handle h, double *mass);                      /* m_mass */
+otfem_status otfem_get_volumes(otfem_handle h, double *V);                      /* m_V */
+otfem_status otfem_get_rotations(otfem_handle h, double *Re);  /* [9T] row-major m_Re */
+otfem_status otfem_get_plastic(otfem_handle h, double *plastic); /* [6T] m_plastic */
+
+/* fem::simulate (fem_simulate.h:49-113): one implicit step.  params NULL means
+ * the reference's settings.  stats may be NULL. */
+otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
+                        const otfem_params *params, otfem_stats *stats);
+/* With stats == NULL otfem_step only enqueues the step on the handle's stream
+ * (no host synchronisation, as a device-resident caller wants); the results
+ * and any error of the enqueued steps are reported by otfem_synchronize.  No
+ * reference counterpart: fem::simulate is synchronous. */
+otfem_status otfem_synchronize(otfem_handle h, otfem_stats *stats);
+/* Per-kernel HIP-event timing of every step (element kernel, CG kernel),
+ * accumulated until the next otfem_set_timing call.  No reference
+ * counterpart (fem has no timing hooks). */
+otfem_status otfem_set_timing(otfem_handle h, int enable);
+otfem_status otfem_get_timing(otfem_handle h, double *element_ms, double *cg_ms,
+                              long long *steps);
+/* Per-body CG iterations and final relative residual of the last step. */
+otfem_status otfem_body_stats(otfem_handle h, int *iterations, double *rel_residual);
+
+/* fem::update_original_coord (fem_update_original_coord.h:28-33): x0 := x.
+ * As in the reference, otfem_init must be called again afterwards. */
+otfem_status otfem_update_original_coord(otfem_handle h);
+
+/* fem::export_K (fem_export_k.h:23-68) as block-CSR over nodes, from the last
+ * step's rotations: fixed rows hold only an identity diagonal block, fixed
+ * columns are dropped.  Call with blocks == NULL to get *nnz_blocks first.
+ * row_ptr [N+1], cols [nnz], blocks [9*nnz] row-major. */
+otfem_status otfem_export_K(otfem_handle h, int64_t *nnz_blocks, int64_t *row_ptr,
+                            int32_t *cols, double *blocks);
+
+/* Layout facts (no device work): tiles, slots (tet copies) and halo nodes. */
+otfem_status otfem_layout_info(otfem_handle h, int64_t *num_tiles, int64_t *num_slots,
+                               int64_t *num_halo, int64_t *max_tile_nodes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OTFEM_OTFEM_H */

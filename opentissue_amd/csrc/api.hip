@@ -1,0 +1,826 @@
+// otfem C ABI (include/otfem/otfem.h): handle, device memory, launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "layout.h"
+#include "otfem/otfem.h"
+
+using namespace otfem;
+
+namespace {
+constexpr int kBlock = 256;
+constexpr double kBarrierSeconds = 10.0;
+const char *kVersion = "otfem 0.1.0 (gfx950)";
+}  // namespace
+
+struct otfem_solver {
+  Layout L;
+  std::string err;
+  // host copies, caller order
+  std::vector<double> x_init, x0_host, v_init;
+  std::vector<int32_t> tets;
+  std::vector<uint8_t> fixed_host;
+  std::vector<double> young, poisson, density;  // per tet, from the last init
+  std::vector<double> mass_host, vol_host;
+  bool device_ready = false;
+  bool initialized = false;
+  bool plastic_on = false;
+  bool rotations_valid = false;
+  int device = -1;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+  // device buffers
+  double *x = nullptr, *x0 = nullptr, *v = nullptr, *fext = nullptr, *mass = nullptr;
+  double *rhs = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr;
+  float *y = nullptr;
+  uint8_t *fixed = nullptr;
+  TileInfo *tiles = nullptr;
+  int32_t *halo = nullptr;
+  ushort4 *conn = nullptr;
+  int32_t *slot_tet = nullptr;
+  double *rest64 = nullptr, *vd64 = nullptr;
+  float *vd32 = nullptr, *bt = nullptr;
+  double *Re = nullptr, *plastic = nullptr, *plastic_param = nullptr;
+  int32_t *body_tile_begin = nullptr;
+  BodyState *body = nullptr;
+  double *tile_partial = nullptr;
+  unsigned int *body_cnt = nullptr;
+  Control *ctrl = nullptr;
+  int cg_grid = 0;
+  size_t lds_elem = 0, lds_cg = 0;
+  long long spin_limit = 0;
+  std::vector<BodyState> last_body;
+  bool pending = false;  // asynchronous steps enqueued since the last sync
+  double last_elem_ms = 0.0, last_cg_ms = 0.0;
+  double acc_elem_ms = 0.0, acc_cg_ms = 0.0;
+  long long timed_steps = 0;
+  bool timing = false;
+};
+
+namespace {
+
+otfem_status fail(otfem_solver *h, otfem_status st, const std::string &msg) {
+  if (h) h->err = msg;
+  return st;
+}
+
+otfem_status hip_fail(otfem_solver *h, hipError_t e, const char *what) {
+  return fail(h, OTFEM_HIP_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(h, call)                                  \
+  do {                                                   \
+    hipError_t e_ = (call);                              \
+    if (e_ != hipSuccess) return hip_fail((h), e_, #call); \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T **p, size_t n) {
+  if (n == 0) n = 1;
+  return hipMalloc(reinterpret_cast<void **>(p), n * sizeof(T));
+}
+
+template <typename T>
+void dfree(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+void free_device(otfem_solver *h) {
+  dfree(h->x); dfree(h->x0); dfree(h->v); dfree(h->fext); dfree(h->mass);
+  dfree(h->rhs); dfree(h->r); dfree(h->p0); dfree(h->p1); dfree(h->y); dfree(h->fixed);
+  dfree(h->tiles); dfree(h->halo); dfree(h->conn); dfree(h->slot_tet);
+  dfree(h->rest64); dfree(h->vd64); dfree(h->vd32); dfree(h->bt);
+  dfree(h->Re); dfree(h->plastic); dfree(h->plastic_param);
+  dfree(h->body_tile_begin); dfree(h->body); dfree(h->tile_partial); dfree(h->body_cnt);
+  dfree(h->ctrl);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->ev2) (void)hipEventDestroy(h->ev2);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  h->ev0 = h->ev1 = h->ev2 = nullptr;
+  h->stream = nullptr;
+  h->device_ready = false;
+}
+
+// caller order [3N] -> internal order
+void to_internal3(const Layout &L, const double *src, std::vector<double> &dst) {
+  dst.resize(3 * L.N);
+  for (int64_t g = 0; g < L.N; ++g) {
+    int64_t u = L.node_of_internal[g];
+    dst[3 * g] = src[3 * u];
+    dst[3 * g + 1] = src[3 * u + 1];
+    dst[3 * g + 2] = src[3 * u + 2];
+  }
+}
+
+void to_caller3(const Layout &L, const std::vector<double> &src, double *dst) {
+  for (int64_t g = 0; g < L.N; ++g) {
+    int64_t u = L.node_of_internal[g];
+    dst[3 * u] = src[3 * g];
+    dst[3 * u + 1] = src[3 * g + 1];
+    dst[3 * u + 2] = src[3 * g + 2];
+  }
+}
+
+otfem_status ensure_device(otfem_solver *h) {
+  if (h->device_ready) return OTFEM_OK;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    return fail(h, OTFEM_NO_DEVICE, "no HIP device");
+  int dev = 0;
+  HIPCHK(h, hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIPCHK(h, hipGetDeviceProperties(&prop, dev));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(h, OTFEM_NO_DEVICE, std::string("otfem is built for gfx950, device is ") +
+                                        prop.gcnArchName);
+  h->device = dev;
+  const Layout &L = h->L;
+  const int64_t N = L.N, S = (int64_t)L.slot_tet.size(), T = L.T, B = L.B;
+  const int64_t nt = (int64_t)L.tiles.size();
+  HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  HIPCHK(h, hipEventCreate(&h->ev0));
+  HIPCHK(h, hipEventCreate(&h->ev1));
+  HIPCHK(h, hipEventCreate(&h->ev2));
+  HIPCHK(h, dalloc(&h->x, 3 * N));
+  HIPCHK(h, dalloc(&h->x0, 3 * N));
+  HIPCHK(h, dalloc(&h->v, 3 * N));
+  HIPCHK(h, dalloc(&h->fext, 3 * N));
+  HIPCHK(h, dalloc(&h->mass, N));
+  HIPCHK(h, dalloc(&h->rhs, 3 * N));
+  HIPCHK(h, dalloc(&h->r, 3 * N));
+  HIPCHK(h, dalloc(&h->p0, 3 * N));
+  HIPCHK(h, dalloc(&h->p1, 3 * N));
+  HIPCHK(h, dalloc(&h->y, 3 * N));
+  HIPCHK(h, dalloc(&h->fixed, N));
+  HIPCHK(h, dalloc(&h->tiles, nt));
+  HIPCHK(h, dalloc(&h->halo, L.halo.size()));
+  HIPCHK(h, dalloc(&h->conn, S));
+  HIPCHK(h, dalloc(&h->slot_tet, S));
+  HIPCHK(h, dalloc(&h->rest64, 9 * S));
+  HIPCHK(h, dalloc(&h->vd64, 3 * S));
+  HIPCHK(h, dalloc(&h->vd32, 3 * S));
+  HIPCHK(h, dalloc(&h->bt, 9 * S));
+  HIPCHK(h, dalloc(&h->Re, 9 * T));
+  HIPCHK(h, dalloc(&h->plastic, 6 * T));
+  HIPCHK(h, dalloc(&h->plastic_param, 3 * T));
+  HIPCHK(h, dalloc(&h->body_tile_begin, B + 1));
+  HIPCHK(h, dalloc(&h->body, B));
+  HIPCHK(h, dalloc(&h->tile_partial, 2 * nt));
+  HIPCHK(h, dalloc(&h->body_cnt, 2 * B));
+  HIPCHK(h, dalloc(&h->ctrl, 1));
+  hipStream_t s = h->stream;
+  HIPCHK(h, hipMemcpyAsync(h->tiles, L.tiles.data(), nt * sizeof(TileInfo), hipMemcpyHostToDevice, s));
+  if (!L.halo.empty())
+    HIPCHK(h, hipMemcpyAsync(h->halo, L.halo.data(), L.halo.size() * sizeof(int32_t),
+                             hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(h->conn, L.slot_conn.data(), S * sizeof(ushort4), hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(h->slot_tet, L.slot_tet.data(), S * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(h->body_tile_begin, L.body_tile_begin.data(), (B + 1) * sizeof(int32_t),
+                           hipMemcpyHostToDevice, s));
+  std::vector<double> tmp;
+  to_internal3(L, h->x_init.data(), tmp);
+  HIPCHK(h, hipMemcpyAsync(h->x, tmp.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  to_internal3(L, h->x0_host.data(), tmp);
+  HIPCHK(h, hipMemcpyAsync(h->x0, tmp.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  to_internal3(L, h->v_init.data(), tmp);
+  HIPCHK(h, hipMemcpyAsync(h->v, tmp.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  std::vector<uint8_t> fx(N);
+  for (int64_t g = 0; g < N; ++g) fx[g] = h->fixed_host[L.node_of_internal[g]];
+  HIPCHK(h, hipMemcpyAsync(h->fixed, fx.data(), N, hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemsetAsync(h->fext, 0, 3 * N * sizeof(double), s));
+  HIPCHK(h, hipMemsetAsync(h->p0, 0, 3 * N * sizeof(double), s));
+  HIPCHK(h, hipMemsetAsync(h->p1, 0, 3 * N * sizeof(double), s));
+  HIPCHK(h, hipMemsetAsync(h->y, 0, 3 * N * sizeof(float), s));
+  HIPCHK(h, hipMemsetAsync(h->plastic, 0, 6 * T * sizeof(double), s));
+  HIPCHK(h, hipMemsetAsync(h->body_cnt, 0, 2 * B * sizeof(unsigned int), s));
+  HIPCHK(h, hipMemsetAsync(h->ctrl, 0, sizeof(Control), s));
+  HIPCHK(h, hipStreamSynchronize(s));
+  int rate_khz = 0;
+  if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
+      rate_khz <= 0)
+    rate_khz = 100000;
+  h->spin_limit = (long long)(kBarrierSeconds * rate_khz * 1000.0);
+  h->device_ready = true;
+  return OTFEM_OK;
+}
+
+// Per-tet element data, following fem_compute_ke.h:41-77 (d1..d18) and
+// fem_compute_volume.h:27-39.  Gradients are stored with the true sign,
+// gradN_k = -B_k of the reference; Ke is quadratic in B, so it is unchanged.
+struct ElemData {
+  double grad[9];
+  double vd[3];
+  double V;  // signed volume (compute_volume edge form)
+};
+
+ElemData element_data(const double *p0, const double *p1, const double *p2, const double *p3,
+                      double E, double nu) {
+  ElemData e;
+  double d = p0[0], d4 = p0[1], d8 = p0[2];
+  double d1 = p1[0] - d, d5 = p1[1] - d4, d9 = p1[2] - d8;
+  double d2 = p2[0] - d, d6 = p2[1] - d4, d10 = p2[2] - d8;
+  double d3 = p3[0] - d, d7 = p3[1] - d4, d11 = p3[2] - d8;
+  double d12 = (d1 * d6 * d11 + d2 * d7 * d9 + d3 * d5 * d10) - d1 * d7 * d10 - d2 * d5 * d11 -
+               d3 * d6 * d9;
+  double d13 = 1.0 / d12;
+  double d14 = std::fabs(d12 / 6.0);
+  double B1[3] = {(d10 * d7 - d6 * d11) * d13, (d2 * d11 - d10 * d3) * d13, (d6 * d3 - d2 * d7) * d13};
+  double B2[3] = {(d5 * d11 - d9 * d7) * d13, (d9 * d3 - d1 * d11) * d13, (d1 * d7 - d5 * d3) * d13};
+  double B3[3] = {(d9 * d6 - d5 * d10) * d13, (d1 * d10 - d9 * d2) * d13, (d5 * d2 - d1 * d6) * d13};
+  for (int c = 0; c < 3; ++c) {
+    e.grad[c] = -B1[c];
+    e.grad[3 + c] = -B2[c];
+    e.grad[6 + c] = -B3[c];
+  }
+  double d15 = E / (1.0 + nu) / (1.0 - 2 * nu);
+  double d16 = (1.0 - nu) * d15;
+  double d17 = nu * d15;
+  double d18 = E / 2 / (1.0 + nu);
+  e.vd[0] = d16 * d14;
+  e.vd[1] = d17 * d14;
+  e.vd[2] = d18 * d14;
+  // e10 . (e20 % e30) / 6 with operator% of math_vector3.h:193
+  double cx = d6 * d11 - d7 * d10, cy = d3 * d10 - d2 * d11, cz = d2 * d7 - d3 * d6;
+  e.V = (d1 * cx + d5 * cy + d9 * cz) / 6.0;
+  return e;
+}
+
+otfem_status init_common(otfem_solver *h, const double *E, const double *nu, const double *rho,
+                         bool uniform, double c_yield, double c_creep, double c_max) {
+  const Layout &L = h->L;
+  const int64_t T = L.T, S = (int64_t)L.slot_tet.size(), N = L.N;
+  if (!(c_yield >= 0) || !(c_creep >= 0) || !(c_max >= 0))
+    return fail(h, OTFEM_INVALID_ARG, "plastic parameters must be non-negative");
+  h->young.resize(T);
+  h->poisson.resize(T);
+  h->density.resize(T);
+  for (int64_t t = 0; t < T; ++t) {
+    double e = uniform ? E[0] : E[t], n = uniform ? nu[0] : nu[t], r = uniform ? rho[0] : rho[t];
+    // fem_uniform_young.h:32, fem_uniform_poisson.h:32-33, density > 0; nu = 0.5
+    // is rejected as well since it makes D infinite.
+    if (!(e >= 0) || !(n >= 0) || !(n < 0.5) || !(r > 0) || !std::isfinite(e) || !std::isfinite(r))
+      return fail(h, OTFEM_INVALID_ARG, "material out of range (E>=0, 0<=nu<0.5, rho>0)");
+    h->young[t] = e;
+    h->poisson[t] = n;
+    h->density[t] = r;
+  }
+  otfem_status st = ensure_device(h);
+  if (st != OTFEM_OK) return st;
+  std::vector<ElemData> ed(T);
+  h->vol_host.resize(T);
+  int64_t inverted = 0;
+  const double *x0 = h->x0_host.data();
+  for (int64_t t = 0; t < T; ++t) {
+    const int32_t *tt = &h->tets[4 * t];
+    ed[t] = element_data(x0 + 3 * tt[0], x0 + 3 * tt[1], x0 + 3 * tt[2], x0 + 3 * tt[3],
+                         h->young[t], h->poisson[t]);
+    h->vol_host[t] = ed[t].V;
+    if (!(ed[t].V > 0)) ++inverted;
+  }
+  if (inverted)
+    return fail(h, OTFEM_INVERTED_TET,
+                std::to_string(inverted) + " tetrahedra have non-positive rest volume");
+  // fem_compute_mass.h:154-171 in caller order
+  h->mass_host.assign(N, 0.0);
+  for (int64_t i = 0; i < N; ++i)
+    h->mass_host[i] = h->fixed_host[i] ? std::numeric_limits<double>::max() : 0.0;
+  for (int64_t t = 0; t < T; ++t) {
+    double amount = h->density[t] * ed[t].V * .25;
+    for (int k = 0; k < 4; ++k) h->mass_host[h->tets[4 * t + k]] += amount;
+  }
+  std::vector<double> rest(9 * S), vd(3 * S);
+  std::vector<float> vd32(3 * S);
+  for (int64_t s = 0; s < S; ++s) {
+    int64_t t = L.slot_tet[s] & 0x7fffffff;
+    for (int k = 0; k < 9; ++k) rest[k * S + s] = ed[t].grad[k];
+    for (int k = 0; k < 3; ++k) {
+      vd[k * S + s] = ed[t].vd[k];
+      vd32[k * S + s] = (float)ed[t].vd[k];
+    }
+  }
+  std::vector<double> mint(N), pp(3 * T);
+  for (int64_t g = 0; g < N; ++g) mint[g] = h->mass_host[L.node_of_internal[g]];
+  for (int64_t t = 0; t < T; ++t) {
+    pp[3 * t] = c_yield;
+    pp[3 * t + 1] = c_creep;
+    pp[3 * t + 2] = c_max;
+  }
+  h->plastic_on = (c_creep > 0.0 && c_max > 0.0);
+  hipStream_t s = h->stream;
+  HIPCHK(h, hipMemcpyAsync(h->rest64, rest.data(), 9 * S * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(h->vd64, vd.data(), 3 * S * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(h->vd32, vd32.data(), 3 * S * sizeof(float), hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(h->mass, mint.data(), N * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(h->plastic_param, pp.data(), 3 * T * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemsetAsync(h->plastic, 0, 6 * T * sizeof(double), s));
+  HIPCHK(h, hipStreamSynchronize(s));
+
+  h->lds_elem = element_lds_bytes(L.max_tile_nodes, L.max_tile_owned, h->plastic_on ? 1 : 0);
+  h->lds_cg = cg_lds_bytes(L.max_tile_nodes, L.max_tile_owned);
+  hipDeviceProp_t prop;
+  HIPCHK(h, hipGetDeviceProperties(&prop, h->device));
+  if (h->lds_elem > (size_t)prop.sharedMemPerBlock || h->lds_cg > (size_t)prop.sharedMemPerBlock)
+    return fail(h, OTFEM_INVALID_ARG, "tile too large for LDS; lower target_tile_nodes");
+  int per_cu = 0;
+  HIPCHK(h, cg_occupancy(kBlock, h->lds_cg, &per_cu));
+  if (per_cu <= 0) return fail(h, OTFEM_HIP_ERROR, "CG kernel does not fit on a CU");
+  int64_t cap = (int64_t)per_cu * prop.multiProcessorCount;
+  h->cg_grid = (int)std::min<int64_t>(cap, (int64_t)L.tiles.size());
+  h->initialized = true;
+  h->rotations_valid = false;
+  return OTFEM_OK;
+}
+
+StepArgs make_args(otfem_solver *h, double dt, int warp, const otfem_params &P) {
+  StepArgs a{};
+  a.x = h->x; a.x0 = h->x0; a.v = h->v; a.fext = h->fext; a.mass = h->mass; a.fixed = h->fixed;
+  a.rhs = h->rhs; a.r = h->r; a.p0 = h->p0; a.p1 = h->p1; a.y = h->y;
+  a.tiles = h->tiles; a.halo = h->halo; a.conn = h->conn; a.slot_tet = h->slot_tet;
+  a.rest64 = h->rest64; a.vd64 = h->vd64; a.vd32 = h->vd32; a.bt = h->bt;
+  a.Re = h->Re; a.plastic = h->plastic; a.plastic_param = h->plastic_param;
+  a.body_tile_begin = h->body_tile_begin; a.body = h->body; a.tile_partial = h->tile_partial;
+  a.body_cnt = h->body_cnt; a.ctrl = h->ctrl;
+  a.S = (int64_t)h->L.slot_tet.size();
+  a.T = h->L.T;
+  a.ntiles = (int32_t)h->L.tiles.size();
+  a.B = (int32_t)h->L.B;
+  a.dt = dt;
+  a.cM = 1.0 + P.alpha * dt;
+  a.cK = dt * dt + P.beta * dt;
+  a.tol = P.tol;
+  a.tiny = P.tiny;
+  a.tol_mode = P.tol_mode;
+  a.cg_min = P.cg_min_iterations;
+  a.cg_max = P.cg_max_iterations;
+  a.warp = warp ? 1 : 0;
+  a.rotation_mode = P.rotation_mode;
+  a.store_rotations = P.store_rotations ? 1 : 0;
+  a.plastic_on = h->plastic_on ? 1 : 0;
+  a.nblocks = h->cg_grid;
+  a.spin_limit = h->spin_limit;
+  return a;
+}
+
+otfem_status sync_and_check(otfem_solver *h, otfem_stats *stats) {
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->pending = false;
+  Control c;
+  HIPCHK(h, hipMemcpy(&c, h->ctrl, sizeof(Control), hipMemcpyDeviceToHost));
+  if (c.abort) {
+    Control z{};
+    (void)hipMemcpy(h->ctrl, &z, sizeof(Control), hipMemcpyHostToDevice);
+    (void)hipMemset(h->body_cnt, 0, 2 * h->L.B * sizeof(unsigned int));
+    return fail(h, OTFEM_HIP_ERROR, "device grid barrier timed out; step aborted");
+  }
+  h->last_body.resize(h->L.B);
+  HIPCHK(h, hipMemcpy(h->last_body.data(), h->body, h->L.B * sizeof(BodyState),
+                      hipMemcpyDeviceToHost));
+  otfem_status ret = OTFEM_OK;
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    for (const BodyState &b : h->last_body) {
+      stats->iterations = std::max(stats->iterations, (int)b.iterations);
+      stats->bodies_converged += b.converged ? 1 : 0;
+      stats->flags |= b.flags;
+    }
+    stats->bodies_capped = (int)h->L.B - stats->bodies_converged;
+    stats->rr0 = h->last_body[0].rr0;
+    stats->rr = h->last_body[0].rr;
+    stats->bb = h->last_body[0].bb;
+    stats->elapsed_ms = h->last_elem_ms + h->last_cg_ms;
+  }
+  for (const BodyState &b : h->last_body)
+    if (b.flags & 1) ret = fail(h, OTFEM_NONFINITE, "a CG scalar became non-finite");
+  return ret;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *otfem_version(void) { return kVersion; }
+
+const char *otfem_status_string(int status) {
+  switch (status) {
+    case OTFEM_OK: return "ok";
+    case OTFEM_INVALID_ARG: return "invalid argument";
+    case OTFEM_INVERTED_TET: return "inverted tetrahedron";
+    case OTFEM_NONFINITE: return "non-finite value";
+    case OTFEM_NOT_CONVERGED: return "not converged";
+    case OTFEM_HIP_ERROR: return "HIP error";
+    case OTFEM_NO_DEVICE: return "no device";
+    case OTFEM_STATE: return "invalid state";
+    default: return "unknown status";
+  }
+}
+
+otfem_status otfem_default_params(otfem_params *p, int mode) {
+  if (!p) return OTFEM_INVALID_ARG;
+  std::memset(p, 0, sizeof(*p));
+  if (mode == 0) {
+    p->alpha = 2.0;
+    p->beta = 0.0;
+    p->cg_min_iterations = 20;
+    p->cg_max_iterations = 20;
+    p->tol_mode = OTFEM_TOL_ABS_R2;
+    p->tol = 1e-3;
+    p->tiny = 1e-10;
+  } else if (mode == 1) {
+    p->alpha = 0.1;
+    p->beta = 0.001;
+    p->cg_min_iterations = 0;
+    p->cg_max_iterations = 200;
+    p->tol_mode = OTFEM_TOL_REL_B;
+    p->tol = 1e-6;
+    p->tiny = 0.0;
+  } else {
+    return OTFEM_INVALID_ARG;
+  }
+  p->rotation_mode = OTFEM_ROT_GRAM_SCHMIDT;
+  p->warm_start = OTFEM_WARM_PREVIOUS_VELOCITY;
+  p->store_rotations = 0;
+  return OTFEM_OK;
+}
+
+otfem_status otfem_device_count(int *count) {
+  if (!count) return OTFEM_INVALID_ARG;
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *count = c;
+  return OTFEM_OK;
+}
+
+otfem_status otfem_create(const otfem_mesh_desc *d, otfem_handle *out) {
+  if (!d || !out || !d->coord || !d->tets || d->num_nodes <= 0 || d->num_tets < 0)
+    return OTFEM_INVALID_ARG;
+  *out = nullptr;
+  otfem_solver *h = new otfem_solver();
+  const int64_t N = d->num_nodes, T = d->num_tets;
+  h->x_init.assign(d->coord, d->coord + 3 * N);
+  h->x0_host.assign(d->model_coord ? d->model_coord : d->coord,
+                    (d->model_coord ? d->model_coord : d->coord) + 3 * N);
+  if (d->velocity)
+    h->v_init.assign(d->velocity, d->velocity + 3 * N);
+  else
+    h->v_init.assign(3 * N, 0.0);
+  if (d->fixed)
+    h->fixed_host.assign(d->fixed, d->fixed + N);
+  else
+    h->fixed_host.assign(N, 0);
+  for (auto &f : h->fixed_host) f = f ? 1 : 0;
+  h->tets.assign(d->tets, d->tets + 4 * T);
+  for (double c : h->x_init)
+    if (!std::isfinite(c)) {
+      delete h;
+      return OTFEM_INVALID_ARG;
+    }
+  int B = d->num_bodies > 1 ? d->num_bodies : 1;
+  if (B > 1 && (!d->body_node_offsets || !d->body_tet_offsets)) {
+    delete h;
+    return OTFEM_INVALID_ARG;
+  }
+  std::string e = build_layout(N, T, h->x0_host.data(), h->tets.data(), B, d->body_node_offsets,
+                               d->body_tet_offsets, d->target_tile_nodes, 16384, h->L);
+  if (!e.empty()) {
+    delete h;
+    return OTFEM_INVALID_ARG;
+  }
+  *out = h;
+  return OTFEM_OK;
+}
+
+otfem_status otfem_destroy(otfem_handle h) {
+  if (!h) return OTFEM_INVALID_ARG;
+  if (h->device_ready) {
+    (void)hipStreamSynchronize(h->stream);
+    free_device(h);
+  }
+  delete h;
+  return OTFEM_OK;
+}
+
+const char *otfem_last_error(otfem_handle h) { return h ? h->err.c_str() : "null handle"; }
+
+otfem_status otfem_init(otfem_handle h, double young, double poisson, double density,
+                        double c_yield, double c_creep, double c_max) {
+  if (!h) return OTFEM_INVALID_ARG;
+  return init_common(h, &young, &poisson, &density, true, c_yield, c_creep, c_max);
+}
+
+otfem_status otfem_init_materials(otfem_handle h, const double *young, const double *poisson,
+                                  const double *density, double c_yield, double c_creep,
+                                  double c_max) {
+  if (!h || !young || !poisson || !density) return OTFEM_INVALID_ARG;
+  return init_common(h, young, poisson, density, false, c_yield, c_creep, c_max);
+}
+
+otfem_status otfem_set_external_forces(otfem_handle h, const double *f) {
+  if (!h || !f) return OTFEM_INVALID_ARG;
+  otfem_status st = ensure_device(h);
+  if (st) return st;
+  std::vector<double> tmp;
+  to_internal3(h->L, f, tmp);
+  HIPCHK(h, hipMemcpyAsync(h->fext, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return OTFEM_OK;
+}
+
+otfem_status otfem_get_external_forces(otfem_handle h, double *f) {
+  if (!h || !f) return OTFEM_INVALID_ARG;
+  otfem_status st = ensure_device(h);
+  if (st) return st;
+  std::vector<double> tmp(3 * h->L.N);
+  HIPCHK(h, hipMemcpyAsync(tmp.data(), h->fext, tmp.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  to_caller3(h->L, tmp, f);
+  return OTFEM_OK;
+}
+
+otfem_status otfem_set_state(otfem_handle h, const double *x, const double *v) {
+  if (!h) return OTFEM_INVALID_ARG;
+  otfem_status st = ensure_device(h);
+  if (st) return st;
+  std::vector<double> tmp;
+  if (x) {
+    to_internal3(h->L, x, tmp);
+    HIPCHK(h, hipMemcpyAsync(h->x, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  if (v) {
+    to_internal3(h->L, v, tmp);
+    HIPCHK(h, hipMemcpyAsync(h->v, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  return OTFEM_OK;
+}
+
+otfem_status otfem_get_state(otfem_handle h, double *x, double *v) {
+  if (!h) return OTFEM_INVALID_ARG;
+  otfem_status st = ensure_device(h);
+  if (st) return st;
+  std::vector<double> tmp(3 * h->L.N);
+  if (x) {
+    HIPCHK(h, hipMemcpyAsync(tmp.data(), h->x, tmp.size() * sizeof(double), hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    to_caller3(h->L, tmp, x);
+  }
+  if (v) {
+    HIPCHK(h, hipMemcpyAsync(tmp.data(), h->v, tmp.size() * sizeof(double), hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    to_caller3(h->L, tmp, v);
+  }
+  return OTFEM_OK;
+}
+
+otfem_status otfem_set_fixed(otfem_handle h, const unsigned char *fixed) {
+  if (!h || !fixed) return OTFEM_INVALID_ARG;
+  for (int64_t i = 0; i < h->L.N; ++i) h->fixed_host[i] = fixed[i] ? 1 : 0;
+  if (!h->device_ready) return OTFEM_OK;
+  std::vector<uint8_t> fx(h->L.N);
+  for (int64_t g = 0; g < h->L.N; ++g) fx[g] = h->fixed_host[h->L.node_of_internal[g]];
+  HIPCHK(h, hipMemcpyAsync(h->fixed, fx.data(), fx.size(), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return OTFEM_OK;
+}
+
+otfem_status otfem_get_mass(otfem_handle h, double *mass) {
+  if (!h || !mass) return OTFEM_INVALID_ARG;
+  if (!h->initialized) return fail(h, OTFEM_STATE, "call otfem_init first");
+  std::copy(h->mass_host.begin(), h->mass_host.end(), mass);
+  return OTFEM_OK;
+}
+
+otfem_status otfem_get_volumes(otfem_handle h, double *V) {
+  if (!h || !V) return OTFEM_INVALID_ARG;
+  if (!h->initialized) return fail(h, OTFEM_STATE, "call otfem_init first");
+  std::copy(h->vol_host.begin(), h->vol_host.end(), V);
+  return OTFEM_OK;
+}
+
+otfem_status otfem_get_rotations(otfem_handle h, double *Re) {
+  if (!h || !Re) return OTFEM_INVALID_ARG;
+  if (!h->initialized) return fail(h, OTFEM_STATE, "call otfem_init first");
+  if (!h->rotations_valid) {
+    // fem_initialize_stiffness_elements.h:50: Re = I until a warped step stores it
+    for (int64_t t = 0; t < h->L.T; ++t)
+      for (int i = 0; i < 9; ++i) Re[9 * t + i] = (i % 4 == 0) ? 1.0 : 0.0;
+    return OTFEM_OK;
+  }
+  HIPCHK(h, hipMemcpyAsync(Re, h->Re, 9 * h->L.T * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return OTFEM_OK;
+}
+
+otfem_status otfem_get_plastic(otfem_handle h, double *plastic) {
+  if (!h || !plastic) return OTFEM_INVALID_ARG;
+  if (!h->initialized) return fail(h, OTFEM_STATE, "call otfem_init first");
+  HIPCHK(h, hipMemcpyAsync(plastic, h->plastic, 6 * h->L.T * sizeof(double), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return OTFEM_OK;
+}
+
+otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
+                        const otfem_params *params, otfem_stats *stats) {
+  if (!h) return OTFEM_INVALID_ARG;
+  if (!h->initialized) return fail(h, OTFEM_STATE, "call otfem_init before otfem_step");
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(h, OTFEM_INVALID_ARG, "dt must be positive");
+  otfem_params P;
+  if (params)
+    P = *params;
+  else
+    otfem_default_params(&P, 0);
+  if (P.cg_max_iterations < 0 || P.cg_min_iterations < 0 || !(P.tol >= 0) || !(P.tiny >= 0) ||
+      (P.tol_mode != 0 && P.tol_mode != 1) || (P.rotation_mode != 0 && P.rotation_mode != 1) ||
+      !std::isfinite(P.alpha) || !std::isfinite(P.beta))
+    return fail(h, OTFEM_INVALID_ARG, "bad solver parameters");
+  if (P.warm_start != OTFEM_WARM_PREVIOUS_VELOCITY)
+    return fail(h, OTFEM_INVALID_ARG, "warm start model not loaded");
+  StepArgs a = make_args(h, dt, use_stiffness_warping, P);
+  hipStream_t s = h->stream;
+  const bool timed = (stats != nullptr) || h->timing;
+  if (timed) HIPCHK(h, hipEventRecord(h->ev0, s));
+  HIPCHK(h, launch_element(a, kBlock, h->lds_elem, s));
+  if (timed) HIPCHK(h, hipEventRecord(h->ev1, s));
+  HIPCHK(h, launch_cg(a, h->cg_grid, kBlock, h->lds_cg, true, s));
+  if (timed) HIPCHK(h, hipEventRecord(h->ev2, s));
+  h->rotations_valid = (P.store_rotations != 0) && use_stiffness_warping;
+  if (!use_stiffness_warping) h->rotations_valid = false;
+  h->pending = true;
+  if (timed) {
+    HIPCHK(h, hipEventSynchronize(h->ev2));
+    float e1 = 0.f, e2 = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&e1, h->ev0, h->ev1));
+    HIPCHK(h, hipEventElapsedTime(&e2, h->ev1, h->ev2));
+    h->last_elem_ms = e1;
+    h->last_cg_ms = e2;
+    h->acc_elem_ms += e1;
+    h->acc_cg_ms += e2;
+    h->timed_steps++;
+  }
+  if (!stats) return OTFEM_OK;
+  return sync_and_check(h, stats);
+}
+
+otfem_status otfem_synchronize(otfem_handle h, otfem_stats *stats) {
+  if (!h) return OTFEM_INVALID_ARG;
+  if (!h->device_ready) return OTFEM_OK;
+  return sync_and_check(h, stats);
+}
+
+otfem_status otfem_set_timing(otfem_handle h, int enable) {
+  if (!h) return OTFEM_INVALID_ARG;
+  h->timing = enable != 0;
+  h->acc_elem_ms = h->acc_cg_ms = 0.0;
+  h->timed_steps = 0;
+  return OTFEM_OK;
+}
+
+otfem_status otfem_get_timing(otfem_handle h, double *element_ms, double *cg_ms,
+                              long long *steps) {
+  if (!h) return OTFEM_INVALID_ARG;
+  if (element_ms) *element_ms = h->acc_elem_ms;
+  if (cg_ms) *cg_ms = h->acc_cg_ms;
+  if (steps) *steps = h->timed_steps;
+  return OTFEM_OK;
+}
+
+otfem_status otfem_body_stats(otfem_handle h, int *iterations, double *rel_residual) {
+  if (!h) return OTFEM_INVALID_ARG;
+  if (h->last_body.size() != (size_t)h->L.B) return fail(h, OTFEM_STATE, "no completed step");
+  for (int64_t b = 0; b < h->L.B; ++b) {
+    const BodyState &s = h->last_body[b];
+    if (iterations) iterations[b] = s.iterations;
+    if (rel_residual) rel_residual[b] = s.bb > 0 ? std::sqrt(s.rr / s.bb) : 0.0;
+  }
+  return OTFEM_OK;
+}
+
+otfem_status otfem_update_original_coord(otfem_handle h) {
+  if (!h) return OTFEM_INVALID_ARG;
+  otfem_status st = ensure_device(h);
+  if (st) return st;
+  std::vector<double> tmp(3 * h->L.N);
+  HIPCHK(h, hipMemcpyAsync(h->x0, h->x, tmp.size() * sizeof(double), hipMemcpyDeviceToDevice,
+                           h->stream));
+  HIPCHK(h, hipMemcpyAsync(tmp.data(), h->x, tmp.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  to_caller3(h->L, tmp, h->x0_host.data());
+  // fem_update_original_coord.h:19-23: the force offsets are now stale and the
+  // caller must initialise again before the next step.
+  h->initialized = false;
+  return OTFEM_OK;
+}
+
+otfem_status otfem_export_K(otfem_handle h, int64_t *nnz_blocks, int64_t *row_ptr, int32_t *cols,
+                            double *blocks) {
+  if (!h || !nnz_blocks) return OTFEM_INVALID_ARG;
+  if (!h->initialized) return fail(h, OTFEM_STATE, "call otfem_init first");
+  const int64_t N = h->L.N, T = h->L.T;
+  std::vector<double> R(9 * T);
+  if (otfem_get_rotations(h, R.data()) != OTFEM_OK) return OTFEM_HIP_ERROR;
+  // K'_ij = sum_e Re Ke_ij Re^T (fem_stiffness_assembly.h:179-203), rows as
+  // ordered maps like the reference's m_K_row.
+  std::vector<std::map<int32_t, std::array<double, 9>>> rows(N);
+  const double *x0 = h->x0_host.data();
+  for (int64_t t = 0; t < T; ++t) {
+    const int32_t *tt = &h->tets[4 * t];
+    ElemData e = element_data(x0 + 3 * tt[0], x0 + 3 * tt[1], x0 + 3 * tt[2], x0 + 3 * tt[3],
+                              h->young[t], h->poisson[t]);
+    double b[4][3];
+    for (int c = 0; c < 3; ++c) {
+      b[1][c] = e.grad[c];
+      b[2][c] = e.grad[3 + c];
+      b[3][c] = e.grad[6 + c];
+      b[0][c] = -(b[1][c] + b[2][c] + b[3][c]);
+    }
+    const double *Rt = &R[9 * t];
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) {
+        // Ke_ij = D1 b_i b_j^T + D2 b_j b_i^T + (D2 b_i.b_j) I + (D0-D1-2D2) diag(b_i o b_j)
+        double K[9];
+        double bij = b[i][0] * b[j][0] + b[i][1] * b[j][1] + b[i][2] * b[j][2];
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) {
+            double v = e.vd[1] * b[i][r] * b[j][c] + e.vd[2] * b[i][c] * b[j][r];
+            if (r == c) v += e.vd[2] * bij + (e.vd[0] - e.vd[1] - 2 * e.vd[2]) * b[i][r] * b[j][r];
+            K[3 * r + c] = v;
+          }
+        double RK[9], W[9];
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c)
+            RK[3 * r + c] = Rt[3 * r] * K[c] + Rt[3 * r + 1] * K[3 + c] + Rt[3 * r + 2] * K[6 + c];
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c)
+            W[3 * r + c] = RK[3 * r] * Rt[3 * c] + RK[3 * r + 1] * Rt[3 * c + 1] +
+                           RK[3 * r + 2] * Rt[3 * c + 2];
+        auto &blk = rows[tt[i]][tt[j]];
+        for (int k = 0; k < 9; ++k) blk[k] += W[k];
+      }
+  }
+  int64_t nnz = 0;
+  for (int64_t i = 0; i < N; ++i) {
+    if (h->fixed_host[i]) {
+      nnz += 1;
+      continue;
+    }
+    for (auto &kv : rows[i])
+      if (!h->fixed_host[kv.first]) ++nnz;
+  }
+  *nnz_blocks = nnz;
+  if (!blocks) return OTFEM_OK;
+  if (!row_ptr || !cols) return OTFEM_INVALID_ARG;
+  int64_t k = 0;
+  row_ptr[0] = 0;
+  for (int64_t i = 0; i < N; ++i) {
+    if (h->fixed_host[i]) {
+      cols[k] = (int32_t)i;
+      for (int q = 0; q < 9; ++q) blocks[9 * k + q] = (q % 4 == 0) ? 1.0 : 0.0;
+      ++k;
+    } else {
+      for (auto &kv : rows[i]) {
+        if (h->fixed_host[kv.first]) continue;
+        cols[k] = kv.first;
+        for (int q = 0; q < 9; ++q) blocks[9 * k + q] = kv.second[q];
+        ++k;
+      }
+    }
+    row_ptr[i + 1] = k;
+  }
+  return OTFEM_OK;
+}
+
+otfem_status otfem_layout_info(otfem_handle h, int64_t *num_tiles, int64_t *num_slots,
+                               int64_t *num_halo, int64_t *max_tile_nodes) {
+  if (!h) return OTFEM_INVALID_ARG;
+  if (num_tiles) *num_tiles = (int64_t)h->L.tiles.size();
+  if (num_slots) *num_slots = (int64_t)h->L.slot_tet.size();
+  if (num_halo) *num_halo = (int64_t)h->L.halo.size();
+  if (max_tile_nodes) *max_tile_nodes = h->L.max_tile_nodes;
+  return OTFEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,199 @@
+// Owner-computes tile layout (see layout.h).
+#include "layout.h"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <unordered_map>
+
+namespace otfem {
+
+namespace {
+
+// Recursive coordinate bisection of `idx[lo, hi)` into `parts` parts whose
+// sizes differ by at most one; each part's node ids are appended to `out` in
+// order and its size to `sizes`.
+void rcb(std::vector<int32_t> &idx, int64_t lo, int64_t hi, int64_t parts, const double *x,
+         std::vector<int32_t> &part_sizes) {
+  if (parts <= 1 || hi - lo <= 1) {
+    part_sizes.push_back((int32_t)(hi - lo));
+    return;
+  }
+  double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+  for (int64_t i = lo; i < hi; ++i)
+    for (int c = 0; c < 3; ++c) {
+      double v = x[3 * (int64_t)idx[i] + c];
+      mn[c] = std::min(mn[c], v);
+      mx[c] = std::max(mx[c], v);
+    }
+  int ax = 0;
+  for (int c = 1; c < 3; ++c)
+    if (mx[c] - mn[c] > mx[ax] - mn[ax]) ax = c;
+  int64_t left_parts = parts / 2;
+  int64_t n = hi - lo;
+  int64_t mid = lo + (n * left_parts) / parts;
+  std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi,
+                   [&](int32_t a, int32_t b) {
+                     double va = x[3 * (int64_t)a + ax], vb = x[3 * (int64_t)b + ax];
+                     if (va != vb) return va < vb;
+                     return a < b;
+                   });
+  rcb(idx, lo, mid, left_parts, x, part_sizes);
+  rcb(idx, mid, hi, parts - left_parts, x, part_sizes);
+}
+
+}  // namespace
+
+std::string build_layout(int64_t N, int64_t T, const double *coords, const int32_t *tets,
+                         int64_t B, const int64_t *body_node_off, const int64_t *body_tet_off,
+                         int target_tile_nodes, int max_tile_nodes, Layout &L) {
+  L = Layout();
+  if (N <= 0 || T < 0) return "num_nodes must be positive and num_tets non-negative";
+  if (N >= (int64_t)INT32_MAX || T >= (int64_t)INT32_MAX) return "mesh too large for 32-bit ids";
+  if (target_tile_nodes <= 0) target_tile_nodes = 512;
+  if (max_tile_nodes <= 0) max_tile_nodes = 8192;
+  L.N = N;
+  L.T = T;
+  if (B <= 1) {
+    L.B = 1;
+    L.body_node_begin = {0, N};
+    L.body_tet_begin = {0, T};
+  } else {
+    L.B = B;
+    L.body_node_begin.assign(body_node_off, body_node_off + B + 1);
+    L.body_tet_begin.assign(body_tet_off, body_tet_off + B + 1);
+    if (L.body_node_begin[0] != 0 || L.body_node_begin[B] != N || L.body_tet_begin[0] != 0 ||
+        L.body_tet_begin[B] != T)
+      return "body offsets must start at 0 and end at num_nodes / num_tets";
+    for (int64_t b = 0; b < B; ++b)
+      if (L.body_node_begin[b + 1] <= L.body_node_begin[b] ||
+          L.body_tet_begin[b + 1] < L.body_tet_begin[b])
+        return "body offsets must be increasing (every body needs a node)";
+  }
+  for (int64_t t = 0; t < T; ++t)
+    for (int k = 0; k < 4; ++k) {
+      int64_t v = tets[4 * t + k];
+      if (v < 0 || v >= N) return "tetrahedron node index out of range";
+    }
+  // tets must stay inside their body
+  if (L.B > 1) {
+    for (int64_t b = 0; b < L.B; ++b)
+      for (int64_t t = L.body_tet_begin[b]; t < L.body_tet_begin[b + 1]; ++t)
+        for (int k = 0; k < 4; ++k) {
+          int64_t v = tets[4 * t + k];
+          if (v < L.body_node_begin[b] || v >= L.body_node_begin[b + 1])
+            return "a tetrahedron references a node of another body";
+        }
+  }
+
+  // 1. Partition nodes of each body into tiles (internal ids = tile-major).
+  L.node_of_internal.resize(N);
+  L.internal_of_node.assign(N, -1);
+  std::vector<int32_t> tile_of_internal(N);
+  L.body_tile_begin.assign(L.B + 1, 0);
+  int64_t next_internal = 0;
+  for (int64_t b = 0; b < L.B; ++b) {
+    int64_t n0 = L.body_node_begin[b], n1 = L.body_node_begin[b + 1];
+    std::vector<int32_t> idx(n1 - n0);
+    std::iota(idx.begin(), idx.end(), (int32_t)n0);
+    int64_t parts = std::max<int64_t>(1, (n1 - n0 + target_tile_nodes - 1) / target_tile_nodes);
+    std::vector<int32_t> sizes;
+    rcb(idx, 0, (int64_t)idx.size(), parts, coords, sizes);
+    int64_t pos = 0;
+    L.body_tile_begin[b] = (int32_t)L.tiles.size();
+    int32_t rank = 0;
+    for (int32_t s : sizes) {
+      TileInfo ti{};
+      ti.node_begin = (int32_t)next_internal;
+      ti.n_owned = s;
+      ti.body = (int32_t)b;
+      ti.body_rank = rank++;
+      // keep user order inside a tile (locality of generator order)
+      std::sort(idx.begin() + pos, idx.begin() + pos + s);
+      for (int32_t k = 0; k < s; ++k) {
+        int32_t u = idx[pos + k];
+        L.node_of_internal[next_internal] = u;
+        L.internal_of_node[u] = (int32_t)next_internal;
+        tile_of_internal[next_internal] = (int32_t)L.tiles.size();
+        ++next_internal;
+      }
+      pos += s;
+      L.tiles.push_back(ti);
+    }
+  }
+  L.body_tile_begin[L.B] = (int32_t)L.tiles.size();
+  const int64_t ntiles = (int64_t)L.tiles.size();
+
+  // 2. Slots: each tet goes into every distinct tile owning one of its corners.
+  std::vector<int64_t> tile_nslots(ntiles, 0);
+  for (int64_t t = 0; t < T; ++t) {
+    int32_t tl[4];
+    for (int k = 0; k < 4; ++k) tl[k] = tile_of_internal[L.internal_of_node[tets[4 * t + k]]];
+    for (int k = 0; k < 4; ++k) {
+      bool dup = false;
+      for (int q = 0; q < k; ++q) dup |= (tl[q] == tl[k]);
+      if (!dup) tile_nslots[tl[k]]++;
+    }
+  }
+  int64_t S = 0;
+  for (int64_t i = 0; i < ntiles; ++i) {
+    if (S + tile_nslots[i] >= (int64_t)INT32_MAX) return "too many slots for 32-bit offsets";
+    L.tiles[i].slot_begin = (int32_t)S;
+    L.tiles[i].n_slots = 0;
+    S += tile_nslots[i];
+  }
+  L.slot_conn.assign(4 * S, 0);
+  L.slot_tet.assign(S, 0);
+  std::vector<int32_t> slot_corner_internal(4 * S);
+  for (int64_t t = 0; t < T; ++t) {
+    int32_t in[4], tl[4];
+    for (int k = 0; k < 4; ++k) {
+      in[k] = L.internal_of_node[tets[4 * t + k]];
+      tl[k] = tile_of_internal[in[k]];
+    }
+    for (int k = 0; k < 4; ++k) {
+      bool dup = false;
+      for (int q = 0; q < k; ++q) dup |= (tl[q] == tl[k]);
+      if (dup) continue;
+      TileInfo &ti = L.tiles[tl[k]];
+      int64_t s = ti.slot_begin + ti.n_slots++;
+      L.slot_tet[s] = (int32_t)t | (k == 0 ? (int32_t)0x80000000u : 0);
+      for (int c = 0; c < 4; ++c) slot_corner_internal[4 * s + c] = in[c];
+    }
+  }
+
+  // 3. Halo tables and tile-local corner indices.
+  std::unordered_map<int32_t, int32_t> local;
+  for (int64_t i = 0; i < ntiles; ++i) {
+    TileInfo &ti = L.tiles[i];
+    ti.halo_begin = (int32_t)L.halo.size();
+    local.clear();
+    for (int64_t s = ti.slot_begin; s < ti.slot_begin + ti.n_slots; ++s)
+      for (int c = 0; c < 4; ++c) {
+        int32_t g = slot_corner_internal[4 * s + c];
+        int32_t li;
+        if (g >= ti.node_begin && g < ti.node_begin + ti.n_owned) {
+          li = g - ti.node_begin;
+        } else {
+          auto it = local.find(g);
+          if (it == local.end()) {
+            li = ti.n_owned + (int32_t)(L.halo.size() - ti.halo_begin);
+            local.emplace(g, li);
+            L.halo.push_back(g);
+          } else {
+            li = it->second;
+          }
+        }
+        if (li > 65535) return "tile node table exceeds 16-bit local indices";
+        L.slot_conn[4 * s + c] = (uint16_t)li;
+      }
+    ti.n_halo = (int32_t)(L.halo.size() - ti.halo_begin);
+    L.max_tile_nodes = std::max(L.max_tile_nodes, ti.n_owned + ti.n_halo);
+    L.max_tile_owned = std::max(L.max_tile_owned, ti.n_owned);
+    L.max_tile_slots = std::max(L.max_tile_slots, ti.n_slots);
+  }
+  if (L.max_tile_nodes > max_tile_nodes) return "tile node table exceeds the LDS budget";
+  return std::string();
+}
+
+}  // namespace otfem

@@ -1,0 +1,55 @@
+// Host-side device layout of a mesh (or CSR-of-bodies batch) for the otfem
+// kernels.  No HIP calls here: built on the CPU at otfem_create.
+//
+// Layout ("owner-computes tiles"):
+//   * Nodes are split per body into tiles by recursive coordinate bisection of
+//     the rest coordinates; a tile OWNS a contiguous range of internal node ids.
+//   * Every tetrahedron is copied into each tile that owns at least one of its
+//     corners; such a copy is a SLOT.  A slot carries the four corners as
+//     tile-local indices into the tile's node table [owned..., halo...].
+//   * A tile computes element terms for all its slots but accumulates only
+//     into its owned nodes, so every node sum is complete inside one
+//     workgroup: no global atomics, deterministic results, and per-node
+//     epilogues (dot products, axpys) fuse into the same pass.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace otfem {
+
+struct TileInfo {
+  int32_t node_begin;  // first owned internal node id
+  int32_t n_owned;
+  int32_t halo_begin;  // offset into halo[]
+  int32_t n_halo;
+  int32_t slot_begin;
+  int32_t n_slots;
+  int32_t body;
+  int32_t body_rank;   // index of this tile within its body
+};
+
+struct Layout {
+  int64_t N = 0, T = 0, B = 0;
+  std::vector<int32_t> node_of_internal;  // internal -> user node id
+  std::vector<int32_t> internal_of_node;  // user -> internal node id
+  std::vector<TileInfo> tiles;
+  std::vector<int32_t> halo;              // internal node ids
+  std::vector<uint16_t> slot_conn;        // 4 per slot, tile-local indices
+  std::vector<int32_t> slot_tet;          // user tet id; bit 31 = primary copy
+  std::vector<int32_t> body_tile_begin;   // B+1
+  std::vector<int64_t> body_node_begin;   // B+1, user node offsets
+  std::vector<int64_t> body_tet_begin;    // B+1, user tet offsets
+  int32_t max_tile_nodes = 0;             // max n_owned + n_halo
+  int32_t max_tile_owned = 0;
+  int32_t max_tile_slots = 0;
+};
+
+// Builds the layout.  `coords` are rest coordinates [3N] (used only for the
+// spatial partition), `tets` [4T].  Bodies: B == 0/1 means one body spanning
+// everything.  Returns an empty string on success, else an error message.
+std::string build_layout(int64_t N, int64_t T, const double *coords, const int32_t *tets,
+                         int64_t B, const int64_t *body_node_off, const int64_t *body_tet_off,
+                         int target_tile_nodes, int max_tile_nodes, Layout &out);
+
+}  // namespace otfem

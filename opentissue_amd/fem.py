@@ -1,0 +1,412 @@
+"""Python mirror of the OpenTissue::fem API over the otfem C ABI.
+
+Two layers:
+
+* :class:`Solver` — a handle on the device solver (``otfem_create`` ...),
+  device-resident state, asynchronous steps.  This is the fast path.
+* :class:`Mesh` with :func:`init`, :func:`simulate`,
+  :func:`update_original_coord` and :func:`export_K` — the reference's
+  ``fem::`` call shapes (fem_init.h:45-54, fem_simulate.h:49-54,
+  fem_update_original_coord.h:28-29, fem_export_k.h:23-24): the caller edits
+  node arrays on the host between steps, and every call marshals them through
+  the ABI, exactly as a C++ caller of ``<otfem/fem.hpp>`` does.
+
+The shared library is ``opentissue_amd/libotfem.so`` (built by
+``__graft_entry__.build()``).  There is no fallback: if it is missing or the
+device is not a gfx950, these calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libotfem.so")
+_lib = None
+
+OK, INVALID_ARG, INVERTED_TET, NONFINITE, NOT_CONVERGED, HIP_ERROR, NO_DEVICE, STATE = range(8)
+
+
+class OtfemError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"otfem status {status}: {message}")
+        self.status = status
+
+
+class Params(C.Structure):
+    """otfem_params (include/otfem/otfem.h)."""
+    _fields_ = [
+        ("alpha", C.c_double),
+        ("beta", C.c_double),
+        ("cg_min_iterations", C.c_int),
+        ("cg_max_iterations", C.c_int),
+        ("tol_mode", C.c_int),
+        ("tol", C.c_double),
+        ("tiny", C.c_double),
+        ("rotation_mode", C.c_int),
+        ("warm_start", C.c_int),
+        ("store_rotations", C.c_int),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int),
+        ("bodies_converged", C.c_int),
+        ("bodies_capped", C.c_int),
+        ("flags", C.c_int),
+        ("rr0", C.c_double),
+        ("rr", C.c_double),
+        ("bb", C.c_double),
+        ("elapsed_ms", C.c_double),
+    ]
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [
+        ("num_nodes", C.c_int),
+        ("num_tets", C.c_int),
+        ("coord", C.POINTER(C.c_double)),
+        ("model_coord", C.POINTER(C.c_double)),
+        ("velocity", C.POINTER(C.c_double)),
+        ("fixed", C.POINTER(C.c_ubyte)),
+        ("tets", C.POINTER(C.c_int32)),
+        ("num_bodies", C.c_int),
+        ("body_node_offsets", C.POINTER(C.c_int64)),
+        ("body_tet_offsets", C.POINTER(C.c_int64)),
+        ("target_tile_nodes", C.c_int),
+    ]
+
+
+# Every symbol of include/otfem/otfem.h, with (restype, argtypes).
+_dp = C.POINTER(C.c_double)
+_H = C.c_void_p
+SIGNATURES = {
+    "otfem_version": (C.c_char_p, []),
+    "otfem_status_string": (C.c_char_p, [C.c_int]),
+    "otfem_default_params": (C.c_int, [C.POINTER(Params), C.c_int]),
+    "otfem_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "otfem_create": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]),
+    "otfem_destroy": (C.c_int, [_H]),
+    "otfem_last_error": (C.c_char_p, [_H]),
+    "otfem_init": (C.c_int, [_H] + [C.c_double] * 6),
+    "otfem_init_materials": (C.c_int, [_H, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double]),
+    "otfem_set_external_forces": (C.c_int, [_H, _dp]),
+    "otfem_get_external_forces": (C.c_int, [_H, _dp]),
+    "otfem_set_state": (C.c_int, [_H, _dp, _dp]),
+    "otfem_get_state": (C.c_int, [_H, _dp, _dp]),
+    "otfem_set_fixed": (C.c_int, [_H, C.POINTER(C.c_ubyte)]),
+    "otfem_get_mass": (C.c_int, [_H, _dp]),
+    "otfem_get_volumes": (C.c_int, [_H, _dp]),
+    "otfem_get_rotations": (C.c_int, [_H, _dp]),
+    "otfem_get_plastic": (C.c_int, [_H, _dp]),
+    "otfem_step": (C.c_int, [_H, C.c_double, C.c_int, C.POINTER(Params), C.POINTER(Stats)]),
+    "otfem_synchronize": (C.c_int, [_H, C.POINTER(Stats)]),
+    "otfem_set_timing": (C.c_int, [_H, C.c_int]),
+    "otfem_get_timing": (C.c_int, [_H, _dp, _dp, C.POINTER(C.c_longlong)]),
+    "otfem_body_stats": (C.c_int, [_H, C.POINTER(C.c_int), _dp]),
+    "otfem_update_original_coord": (C.c_int, [_H]),
+    "otfem_export_K": (C.c_int, [_H, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                 C.POINTER(C.c_int32), _dp]),
+    "otfem_layout_info": (C.c_int, [_H] + [C.POINTER(C.c_int64)] * 4),
+}
+
+
+def load_library():
+    """Load libotfem.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OtfemError(NO_DEVICE, f"{LIB_PATH} is missing: run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def default_params(mode=0) -> Params:
+    p = Params()
+    load_library().otfem_default_params(C.byref(p), mode)
+    return p
+
+
+def reference_params() -> Params:
+    """The reference's hard-coded settings: alpha 2, beta 0, 20 CG iterations."""
+    return default_params(0)
+
+
+def config_params(alpha=0.1, beta=0.001, tol=1e-6, cap=200, rotation_mode=0, store_rotations=0) -> Params:
+    p = default_params(1)
+    p.alpha, p.beta, p.tol, p.cg_max_iterations = alpha, beta, tol, cap
+    p.rotation_mode, p.store_rotations = rotation_mode, store_rotations
+    return p
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a, shape):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(shape))
+
+
+class Solver:
+    """Device solver for one mesh or a CSR-of-bodies batch."""
+
+    def __init__(self, coords, tets, fixed=None, velocity=None, model_coords=None,
+                 body_node_offsets=None, body_tet_offsets=None, target_tile_nodes=0):
+        self.lib = load_library()
+        coords = _f64(coords, (-1, 3))
+        self.N = coords.shape[0]
+        tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+        self.T = tets.shape[0]
+        mc = None if model_coords is None else _f64(model_coords, (self.N, 3))
+        vel = None if velocity is None else _f64(velocity, (self.N, 3))
+        fx = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.uint8)
+        d = MeshDesc()
+        d.num_nodes, d.num_tets = self.N, self.T
+        d.coord = _dptr(coords)
+        d.model_coord = _dptr(mc) if mc is not None else None
+        d.velocity = _dptr(vel) if vel is not None else None
+        d.fixed = fx.ctypes.data_as(C.POINTER(C.c_ubyte)) if fx is not None else None
+        d.tets = tets.ctypes.data_as(C.POINTER(C.c_int32))
+        self.B = 1
+        if body_node_offsets is not None:
+            bno = np.ascontiguousarray(body_node_offsets, dtype=np.int64)
+            bto = np.ascontiguousarray(body_tet_offsets, dtype=np.int64)
+            self.B = len(bno) - 1
+            d.num_bodies = self.B
+            d.body_node_offsets = bno.ctypes.data_as(C.POINTER(C.c_int64))
+            d.body_tet_offsets = bto.ctypes.data_as(C.POINTER(C.c_int64))
+            self._keep_b = (bno, bto)
+        d.target_tile_nodes = int(target_tile_nodes)
+        h = C.c_void_p()
+        st = self.lib.otfem_create(C.byref(d), C.byref(h))
+        if st != OK:
+            raise OtfemError(st, "otfem_create failed (invalid mesh description)")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.otfem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st not in (OK,):
+            msg = self.lib.otfem_last_error(self.h)
+            raise OtfemError(st, msg.decode() if msg else "")
+        return st
+
+    # fem::init
+    def init(self, young=5e5, poisson=0.33, density=1000.0, c_yield=0.0, c_creep=0.0, c_max=0.0):
+        self._check(self.lib.otfem_init(self.h, young, poisson, density, c_yield, c_creep, c_max))
+
+    def init_materials(self, young, poisson, density, c_yield=0.0, c_creep=0.0, c_max=0.0):
+        E = _f64(np.broadcast_to(young, (self.T,)), (self.T,))
+        nu = _f64(np.broadcast_to(poisson, (self.T,)), (self.T,))
+        rho = _f64(np.broadcast_to(density, (self.T,)), (self.T,))
+        self._check(self.lib.otfem_init_materials(self.h, _dptr(E), _dptr(nu), _dptr(rho),
+                                                  c_yield, c_creep, c_max))
+
+    def set_external_forces(self, f):
+        f = _f64(f, (self.N, 3))
+        self._check(self.lib.otfem_set_external_forces(self.h, _dptr(f)))
+
+    def external_forces(self):
+        f = np.empty((self.N, 3))
+        self._check(self.lib.otfem_get_external_forces(self.h, _dptr(f)))
+        return f
+
+    def set_state(self, x=None, v=None):
+        xp = _f64(x, (self.N, 3)) if x is not None else None
+        vp = _f64(v, (self.N, 3)) if v is not None else None
+        self._check(self.lib.otfem_set_state(self.h, _dptr(xp) if xp is not None else None,
+                                             _dptr(vp) if vp is not None else None))
+
+    def state(self):
+        x = np.empty((self.N, 3))
+        v = np.empty((self.N, 3))
+        self._check(self.lib.otfem_get_state(self.h, _dptr(x), _dptr(v)))
+        return x, v
+
+    def set_fixed(self, fixed):
+        fx = np.ascontiguousarray(fixed, dtype=np.uint8)
+        self._check(self.lib.otfem_set_fixed(self.h, fx.ctypes.data_as(C.POINTER(C.c_ubyte))))
+
+    def mass(self):
+        m = np.empty(self.N)
+        self._check(self.lib.otfem_get_mass(self.h, _dptr(m)))
+        return m
+
+    def volumes(self):
+        V = np.empty(self.T)
+        self._check(self.lib.otfem_get_volumes(self.h, _dptr(V)))
+        return V
+
+    def rotations(self):
+        R = np.empty((self.T, 3, 3))
+        self._check(self.lib.otfem_get_rotations(self.h, _dptr(R)))
+        return R
+
+    def plastic(self):
+        p = np.empty((self.T, 6))
+        self._check(self.lib.otfem_get_plastic(self.h, _dptr(p)))
+        return p
+
+    # fem::simulate
+    def simulate(self, dt, warp=True, params=None):
+        P = reference_params() if params is None else params
+        st = Stats()
+        self._check(self.lib.otfem_step(self.h, dt, 1 if warp else 0, C.byref(P), C.byref(st)))
+        return st
+
+    def step_async(self, dt, warp=True, params=None):
+        P = reference_params() if params is None else params
+        self._check(self.lib.otfem_step(self.h, dt, 1 if warp else 0, C.byref(P), None))
+
+    def synchronize(self):
+        st = Stats()
+        self._check(self.lib.otfem_synchronize(self.h, C.byref(st)))
+        return st
+
+    def set_timing(self, enable=True):
+        self._check(self.lib.otfem_set_timing(self.h, 1 if enable else 0))
+
+    def timing(self):
+        e = C.c_double()
+        c = C.c_double()
+        n = C.c_longlong()
+        self._check(self.lib.otfem_get_timing(self.h, C.byref(e), C.byref(c), C.byref(n)))
+        return e.value, c.value, n.value
+
+    def body_stats(self):
+        it = np.empty(self.B, dtype=np.int32)
+        rel = np.empty(self.B)
+        self._check(self.lib.otfem_body_stats(self.h, it.ctypes.data_as(C.POINTER(C.c_int)), _dptr(rel)))
+        return it, rel
+
+    def update_original_coord(self):
+        self._check(self.lib.otfem_update_original_coord(self.h))
+
+    def export_K(self):
+        """Block-CSR (row_ptr, cols, blocks[nnz,3,3]) of fem::export_K."""
+        nnz = C.c_int64()
+        self._check(self.lib.otfem_export_K(self.h, C.byref(nnz), None, None, None))
+        rp = np.empty(self.N + 1, dtype=np.int64)
+        cols = np.empty(nnz.value, dtype=np.int32)
+        blocks = np.empty((nnz.value, 3, 3))
+        self._check(self.lib.otfem_export_K(
+            self.h, C.byref(nnz), rp.ctypes.data_as(C.POINTER(C.c_int64)),
+            cols.ctypes.data_as(C.POINTER(C.c_int32)), _dptr(blocks)))
+        return rp, cols, blocks
+
+    def layout_info(self):
+        vals = [C.c_int64() for _ in range(4)]
+        self._check(self.lib.otfem_layout_info(self.h, *[C.byref(v) for v in vals]))
+        return dict(tiles=vals[0].value, slots=vals[1].value, halo=vals[2].value,
+                    max_tile_nodes=vals[3].value)
+
+
+# ---------------------------------------------------------------------------
+# Reference-shaped API: fem::Mesh + free functions.
+
+
+@dataclass
+class Mesh:
+    """Host mirror of ``fem::Mesh`` node/tet trait fields used by a caller
+    (fem_node_traits.h:37-52, fem_tetrahedron_traits.h:29-35)."""
+    coord: np.ndarray            # m_coord       [N,3]
+    tets: np.ndarray             # i, j, k, m    [T,4]
+    model_coord: np.ndarray = None   # m_model_coord
+    velocity: np.ndarray = None      # m_velocity
+    fixed: np.ndarray = None         # m_fixed
+    f_external: np.ndarray = None    # m_f_external
+    young: np.ndarray = None         # m_young (per tet), for a custom init
+    poisson: np.ndarray = None
+    density: np.ndarray = None
+    mass: np.ndarray = None          # m_mass (filled by init)
+    Re: np.ndarray = None            # m_Re (filled by simulate when warping)
+    solver: Solver = None
+
+    def __post_init__(self):
+        self.coord = np.array(self.coord, dtype=np.float64).reshape(-1, 3)
+        self.tets = np.ascontiguousarray(self.tets, dtype=np.int32).reshape(-1, 4)
+        n = self.coord.shape[0]
+        if self.model_coord is None:
+            self.model_coord = self.coord.copy()
+        if self.velocity is None:
+            self.velocity = np.zeros((n, 3))
+        if self.fixed is None:
+            self.fixed = np.zeros(n, dtype=bool)
+        if self.f_external is None:
+            self.f_external = np.zeros((n, 3))
+
+    def size_nodes(self):
+        return self.coord.shape[0]
+
+    def size_tetrahedra(self):
+        return self.tets.shape[0]
+
+
+def init(mesh: Mesh, young, poisson, density, c_yield=0.0, c_creep=0.0, c_max=0.0):
+    """fem::init (fem_init.h:45-65); per-tet materials if mesh.young etc. are set
+    and young is None (the user-written init of fem_init.h:32-35)."""
+    if mesh.solver is not None:
+        mesh.solver.close()
+    mesh.solver = Solver(mesh.coord, mesh.tets, mesh.fixed, mesh.velocity, mesh.model_coord)
+    if young is None:
+        mesh.solver.init_materials(mesh.young, mesh.poisson, mesh.density, c_yield, c_creep, c_max)
+    else:
+        mesh.solver.init(young, poisson, density, c_yield, c_creep, c_max)
+    mesh.mass = mesh.solver.mass()
+
+
+def simulate(mesh: Mesh, time_step, use_stiffness_warping=True, params=None):
+    """fem::simulate (fem_simulate.h:49-113).  Host fields are uploaded, one
+    step is taken, and coord / velocity / f_external / Re are read back."""
+    if mesh.solver is None:
+        raise OtfemError(STATE, "fem.init must be called before fem.simulate")
+    s = mesh.solver
+    s.set_fixed(mesh.fixed)
+    s.set_state(mesh.coord, mesh.velocity)
+    s.set_external_forces(mesh.f_external)
+    P = reference_params() if params is None else params
+    P.store_rotations = 1
+    st = s.simulate(time_step, use_stiffness_warping, P)
+    mesh.coord[:], mesh.velocity[:] = s.state()
+    mesh.f_external[:] = s.external_forces()
+    mesh.Re = s.rotations()
+    return st
+
+
+def update_original_coord(mesh: Mesh):
+    """fem::update_original_coord (fem_update_original_coord.h:28-33)."""
+    mesh.model_coord[:] = mesh.coord
+    if mesh.solver is not None:
+        mesh.solver.set_state(mesh.coord, None)
+        mesh.solver.update_original_coord()
+
+
+def export_K(mesh: Mesh):
+    """fem::export_K (fem_export_k.h:23-68) as a dense 3N x 3N array."""
+    rp, cols, blocks = mesh.solver.export_K()
+    N = mesh.size_nodes()
+    K = np.zeros((3 * N, 3 * N))
+    for i in range(N):
+        for k in range(rp[i], rp[i + 1]):
+            j = cols[k]
+            K[3 * i:3 * i + 3, 3 * j:3 * j + 3] = blocks[k]
+    return K
