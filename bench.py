@@ -1,0 +1,236 @@
+#!/usr/bin/env python3
+"""Throughput of the corotated tetrahedral FEM step on one MI355X.
+
+Default workload: the 64^3-cube lattice (1,310,720 tets, 274,625 nodes),
+interior nodes jittered +/-0.2 h, E log-uniform per tet in [1e5, 1e6],
+nu 0.3, face x = 0 fixed, 1e3 Pa traction on the opposite face, dt 0.005,
+CG to relative residual 1e-6 (cap 200), Rayleigh alpha 0.1 / beta 0.001.
+A step is one full implicit step (rotations + elastic forces + right-hand
+side + CG + position update) over the whole mesh, inputs resident in HBM.
+
+Output: one JSON line.  `value` is tet-steps/s over exactly --steps timed
+steps from rest; `roofline` prices the dominant kernel (the persistent CG
+kernel) by its algorithmic bytes (100 B/tet + 195 B/node per CG iteration,
+FP64 v and r, plus 36 B/node for the position update) against the MI355X HBM
+peak; `cpu_baseline` times the FP64 CPU restatement of the reference step
+(one thread) on a bounded sample of the same workload.
+
+    python bench.py [--steps 200] [--warmup 5] [--config cube64|cantilever|bodies]
+    python bench.py --copy-probe     # plain device copy of known size, for
+                                     # scaling rocprofv3 FETCH/WRITE counters
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_TET_ITER = 100
+BYTES_PER_NODE_ITER = 195
+BYTES_PER_NODE_POSUPDATE = 36
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_info():
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def build_workload(name):
+    from opentissue_amd import workloads
+    if name == "cube64":
+        return workloads.cube64()
+    if name == "cantilever":
+        return workloads.cantilever()
+    if name == "bodies":
+        return workloads.bodies(512, 4, 24)
+    raise SystemExit(f"unknown config {name}")
+
+
+def copy_probe():
+    import torch
+    n = 1 << 28  # 2^28 float32 = 1 GiB
+    a = torch.empty(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n,
+                      "launches": 5}))
+
+
+def run_cpu_baseline(w, params_kw, seconds_budget):
+    """One implicit step of the same workload from rest with the FP64 oracle."""
+    import oracle
+    t_init0 = time.perf_counter()
+    o = oracle.Oracle(w.coords, w.tets, w.fixed)
+    o.init_materials(w.young, w.poisson, w.density)
+    o.set_external_forces(w.forces)
+    t_init = time.perf_counter() - t_init0
+    P = oracle.config_params(**params_kw)
+    t0 = time.perf_counter()
+    st = o.simulate(w.dt, True, P)
+    t = time.perf_counter() - t0
+    x, v = o.state()
+    return dict(seconds=t, init_seconds=t_init, iterations=st.iterations, x=x, v=v)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="cube64")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=60.0)
+    ap.add_argument("--tile-nodes", type=int, default=0)
+    ap.add_argument("--copy-probe", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU "
+              "sharding of this path is a design note, not a measured configuration.",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
+
+    from opentissue_amd import fem
+
+    w = build_workload(args.config)
+    log(f"workload {w.name}: {w.num_bodies} bodies, {w.num_tets} tets, {w.num_nodes} nodes")
+    s = fem.Solver(w.coords, w.tets, w.fixed, body_node_offsets=w.body_node_offsets,
+                   body_tet_offsets=w.body_tet_offsets, target_tile_nodes=args.tile_nodes)
+    s.init_materials(w.young, w.poisson, w.density)
+    s.set_external_forces(w.forces)
+    params_kw = dict(alpha=0.1, beta=0.001, tol=1e-6, cap=200)
+    P = fem.config_params(**params_kw)
+    layout = s.layout_info()
+
+    # warmup, then back to rest so the timed steps are steps 1..K from rest
+    for _ in range(args.warmup):
+        s.step_async(w.dt, True, P)
+    s.synchronize()
+    zeros = np.zeros_like(w.coords)
+    s.set_state(w.coords, zeros)
+    s.iteration_totals(reset=True)
+    s.set_timing(True)
+    s.synchronize()
+
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.step_async(w.dt, True, P)
+    st = s.synchronize()
+    t1 = time.perf_counter()
+    elem_ms, cg_ms, nsteps = s.timing()
+    iters = s.iteration_totals()
+    wall = t1 - t0
+    T, N = w.num_tets, w.num_nodes
+    value = T * args.steps / wall
+    total_iters = int(iters.sum())
+    # algorithmic bytes of all CG launches in the timed region
+    if w.num_bodies == 1:
+        cg_bytes = total_iters * (BYTES_PER_TET_ITER * T + BYTES_PER_NODE_ITER * N)
+    else:
+        bt = np.diff(w.body_tet_offsets)
+        bn = np.diff(w.body_node_offsets)
+        cg_bytes = float(np.sum(iters * (BYTES_PER_TET_ITER * bt + BYTES_PER_NODE_ITER * bn)))
+    cg_bytes += args.steps * BYTES_PER_NODE_POSUPDATE * N
+    achieved = cg_bytes / (cg_ms / 1e3) / 1e9 if cg_ms > 0 else 0.0
+    log(f"timed {args.steps} steps in {wall:.3f} s; element {elem_ms:.2f} ms, CG {cg_ms:.2f} ms; "
+        f"CG iterations {total_iters} ({total_iters / max(1, args.steps) / w.num_bodies:.1f} per step per body)")
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"k_cg_{args.config}_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    result = {
+        "metric": "tet-steps/s at CG rel-res 1e-6 (plain warm start); CG HBM GB/s % of peak",
+        "value": value,
+        "unit": "tet-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{w.name}: {w.num_bodies} body/bodies, {T} tets, {N} nodes, "
+                        f"dt {w.dt}, CG rel-res 1e-6 cap 200, alpha 0.1 beta 0.001, warm start v^i",
+            "tets": T, "nodes": N, "bodies": w.num_bodies,
+            "tiles": layout["tiles"], "slots": layout["slots"],
+        },
+        "cg_iterations_per_step": total_iters / args.steps / w.num_bodies,
+        "kernel_ms_per_step": {"element": elem_ms / max(1, nsteps), "cg": cg_ms / max(1, nsteps)},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "k_cg (persistent CG + position update)",
+            "algorithmic_bytes_per_launch": cg_bytes / args.steps,
+            "avg_launch_ms": cg_ms / max(1, nsteps),
+        },
+        "cpu_baseline": None,
+    }
+
+    if not args.no_cpu:
+        # bounded sample: one step from rest of the same workload; the GPU's
+        # first step from the same state is the parity check at full size
+        s2 = fem.Solver(w.coords, w.tets, w.fixed, body_node_offsets=w.body_node_offsets,
+                        body_tet_offsets=w.body_tet_offsets, target_tile_nodes=args.tile_nodes)
+        s2.init_materials(w.young, w.poisson, w.density)
+        s2.set_external_forces(w.forces)
+        s2.simulate(w.dt, True, P)
+        xg, _ = s2.state()
+        s2.close()
+        log("cpu baseline: one step of the same workload with the FP64 oracle (1 thread)")
+        cb = run_cpu_baseline(w, params_kw, args.cpu_seconds)
+        model, ncpu = cpu_info()
+        den = np.linalg.norm(cb["x"] - w.coords, axis=1).max()
+        err = np.linalg.norm(xg - cb["x"], axis=1).max() / den if den > 0 else 0.0
+        result["cpu_baseline"] = {
+            "value": T / cb["seconds"],
+            "unit": "tet-steps/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"1 implicit step from rest of the same {w.name} workload "
+                      f"({cb['iterations']} CG iterations), FP64 ordered-map restatement of "
+                      f"fem::simulate, {cb['seconds']:.1f} s on {model} ({ncpu} host cpus, 1 used)",
+        }
+        result["parity_first_step"] = {"rel_displacement_error": err,
+                                       "cg_iterations_cpu": cb["iterations"]}
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -145,6 +145,18 @@ otfem_status otfem_synchronize(otfem_handle h, otfem_stats *stats);
 otfem_status otfem_set_timing(otfem_handle h, int enable);
 otfem_status otfem_get_timing(otfem_handle h, double *element_ms, double *cg_ms,
                               long long *steps);
+/* CG iterations per body summed over all steps since the last reset
+ * (reset != 0 clears after reading; totals may be NULL).  No reference
+ * counterpart; used to count algorithmic bytes of asynchronous steps. */
+otfem_status otfem_iteration_totals(otfem_handle h, int64_t *totals, int reset);
+/* Kernel configuration: threads per tile in the CG kernels (256/512/1024),
+ * SpMV precision inside CG (OTFEM_SPMV_*: FP32 arithmetic with FP32 or FP64
+ * accumulation, FP64 arithmetic on FP32-stored element data (default), or FP64
+ * arithmetic on FP64-stored element data), and whether steps are replayed as
+ * captured hipGraphs (1) or launched kernel by kernel (0).  No reference
+ * counterpart (tuning of this implementation). */
+enum { OTFEM_SPMV_F32 = 0, OTFEM_SPMV_F32_ACC64 = 1, OTFEM_SPMV_F64 = 2, OTFEM_SPMV_F64_STORE64 = 3 };
+otfem_status otfem_set_kernel_config(otfem_handle h, int cg_block, int spmv_mode, int use_graphs);
 /* Per-body CG iterations and final relative residual of the last step. */
 otfem_status otfem_body_stats(otfem_handle h, int *iterations, double *rel_residual);
 

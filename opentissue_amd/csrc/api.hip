@@ -19,9 +19,7 @@
 using namespace otfem;
 
 namespace {
-constexpr int kBlock = 256;
-constexpr double kBarrierSeconds = 10.0;
-const char *kVersion = "otfem 0.1.0 (gfx950)";
+const char *kVersion = "otfem 0.2.0 (gfx950)";
 }  // namespace
 
 struct otfem_solver {
@@ -43,29 +41,36 @@ struct otfem_solver {
   // device buffers
   double *x = nullptr, *x0 = nullptr, *v = nullptr, *fext = nullptr, *mass = nullptr;
   double *rhs = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr;
-  float *y = nullptr;
+  double *y = nullptr;
   uint8_t *fixed = nullptr;
   TileInfo *tiles = nullptr;
   int32_t *halo = nullptr;
   ushort4 *conn = nullptr;
   int32_t *slot_tet = nullptr;
   double *rest64 = nullptr, *vd64 = nullptr;
-  float *vd32 = nullptr, *bt = nullptr;
+  float4 *bt4 = nullptr;
+  double4 *bt8 = nullptr;
   double *Re = nullptr, *plastic = nullptr, *plastic_param = nullptr;
   int32_t *body_tile_begin = nullptr;
   BodyState *body = nullptr;
-  double *tile_partial = nullptr;
-  unsigned int *body_cnt = nullptr;
-  Control *ctrl = nullptr;
-  int cg_grid = 0;
+  double *part_bb = nullptr, *part_rr = nullptr, *part_pq = nullptr;
+  int *done_count = nullptr;
+  unsigned long long *iter_acc = nullptr;
   size_t lds_elem = 0, lds_cg = 0;
-  long long spin_limit = 0;
+  LaunchConfig lc;
+  // captured step graphs (element part, CG part) and the inputs they bake in
+  bool use_graphs = true;
+  hipGraphExec_t g_elem = nullptr, g_cg = nullptr;
+  std::vector<unsigned char> graph_key;
   std::vector<BodyState> last_body;
   bool pending = false;  // asynchronous steps enqueued since the last sync
   double last_elem_ms = 0.0, last_cg_ms = 0.0;
   double acc_elem_ms = 0.0, acc_cg_ms = 0.0;
   long long timed_steps = 0;
   bool timing = false;
+  // per-step event triples of asynchronous timed steps, resolved at sync
+  std::vector<hipEvent_t> pool;
+  size_t pool_used = 0;
 };
 
 namespace {
@@ -101,10 +106,17 @@ void free_device(otfem_solver *h) {
   dfree(h->x); dfree(h->x0); dfree(h->v); dfree(h->fext); dfree(h->mass);
   dfree(h->rhs); dfree(h->r); dfree(h->p0); dfree(h->p1); dfree(h->y); dfree(h->fixed);
   dfree(h->tiles); dfree(h->halo); dfree(h->conn); dfree(h->slot_tet);
-  dfree(h->rest64); dfree(h->vd64); dfree(h->vd32); dfree(h->bt);
+  dfree(h->rest64); dfree(h->vd64); dfree(h->bt4); dfree(h->bt8);
   dfree(h->Re); dfree(h->plastic); dfree(h->plastic_param);
-  dfree(h->body_tile_begin); dfree(h->body); dfree(h->tile_partial); dfree(h->body_cnt);
-  dfree(h->ctrl);
+  dfree(h->body_tile_begin); dfree(h->body); dfree(h->part_bb); dfree(h->part_rr);
+  dfree(h->part_pq); dfree(h->done_count); dfree(h->iter_acc);
+  if (h->g_elem) (void)hipGraphExecDestroy(h->g_elem);
+  if (h->g_cg) (void)hipGraphExecDestroy(h->g_cg);
+  h->g_elem = h->g_cg = nullptr;
+  h->graph_key.clear();
+  for (hipEvent_t e : h->pool) (void)hipEventDestroy(e);
+  h->pool.clear();
+  h->pool_used = 0;
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->ev2) (void)hipEventDestroy(h->ev2);
@@ -171,16 +183,17 @@ otfem_status ensure_device(otfem_solver *h) {
   HIPCHK(h, dalloc(&h->slot_tet, S));
   HIPCHK(h, dalloc(&h->rest64, 9 * S));
   HIPCHK(h, dalloc(&h->vd64, 3 * S));
-  HIPCHK(h, dalloc(&h->vd32, 3 * S));
-  HIPCHK(h, dalloc(&h->bt, 9 * S));
+  HIPCHK(h, dalloc(&h->bt4, 3 * S));
   HIPCHK(h, dalloc(&h->Re, 9 * T));
   HIPCHK(h, dalloc(&h->plastic, 6 * T));
   HIPCHK(h, dalloc(&h->plastic_param, 3 * T));
   HIPCHK(h, dalloc(&h->body_tile_begin, B + 1));
   HIPCHK(h, dalloc(&h->body, B));
-  HIPCHK(h, dalloc(&h->tile_partial, 2 * nt));
-  HIPCHK(h, dalloc(&h->body_cnt, 2 * B));
-  HIPCHK(h, dalloc(&h->ctrl, 1));
+  HIPCHK(h, dalloc(&h->part_bb, nt));
+  HIPCHK(h, dalloc(&h->part_rr, 2 * nt));
+  HIPCHK(h, dalloc(&h->part_pq, nt));
+  HIPCHK(h, dalloc(&h->done_count, 1));
+  HIPCHK(h, dalloc(&h->iter_acc, B));
   hipStream_t s = h->stream;
   HIPCHK(h, hipMemcpyAsync(h->tiles, L.tiles.data(), nt * sizeof(TileInfo), hipMemcpyHostToDevice, s));
   if (!L.halo.empty())
@@ -206,16 +219,15 @@ otfem_status ensure_device(otfem_solver *h) {
   HIPCHK(h, hipMemsetAsync(h->fext, 0, 3 * N * sizeof(double), s));
   HIPCHK(h, hipMemsetAsync(h->p0, 0, 3 * N * sizeof(double), s));
   HIPCHK(h, hipMemsetAsync(h->p1, 0, 3 * N * sizeof(double), s));
-  HIPCHK(h, hipMemsetAsync(h->y, 0, 3 * N * sizeof(float), s));
+  HIPCHK(h, hipMemsetAsync(h->y, 0, 3 * N * sizeof(double), s));
   HIPCHK(h, hipMemsetAsync(h->plastic, 0, 6 * T * sizeof(double), s));
-  HIPCHK(h, hipMemsetAsync(h->body_cnt, 0, 2 * B * sizeof(unsigned int), s));
-  HIPCHK(h, hipMemsetAsync(h->ctrl, 0, sizeof(Control), s));
+  HIPCHK(h, hipMemsetAsync(h->body, 0, B * sizeof(BodyState), s));
+  HIPCHK(h, hipMemsetAsync(h->part_bb, 0, nt * sizeof(double), s));
+  HIPCHK(h, hipMemsetAsync(h->part_rr, 0, 2 * nt * sizeof(double), s));
+  HIPCHK(h, hipMemsetAsync(h->part_pq, 0, nt * sizeof(double), s));
+  HIPCHK(h, hipMemsetAsync(h->done_count, 0, sizeof(int), s));
+  HIPCHK(h, hipMemsetAsync(h->iter_acc, 0, B * sizeof(unsigned long long), s));
   HIPCHK(h, hipStreamSynchronize(s));
-  int rate_khz = 0;
-  if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
-      rate_khz <= 0)
-    rate_khz = 100000;
-  h->spin_limit = (long long)(kBarrierSeconds * rate_khz * 1000.0);
   h->device_ready = true;
   return OTFEM_OK;
 }
@@ -305,14 +317,10 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
     for (int k = 0; k < 4; ++k) h->mass_host[h->tets[4 * t + k]] += amount;
   }
   std::vector<double> rest(9 * S), vd(3 * S);
-  std::vector<float> vd32(3 * S);
   for (int64_t s = 0; s < S; ++s) {
     int64_t t = L.slot_tet[s] & 0x7fffffff;
     for (int k = 0; k < 9; ++k) rest[k * S + s] = ed[t].grad[k];
-    for (int k = 0; k < 3; ++k) {
-      vd[k * S + s] = ed[t].vd[k];
-      vd32[k * S + s] = (float)ed[t].vd[k];
-    }
+    for (int k = 0; k < 3; ++k) vd[k * S + s] = ed[t].vd[k];
   }
   std::vector<double> mint(N), pp(3 * T);
   for (int64_t g = 0; g < N; ++g) mint[g] = h->mass_host[L.node_of_internal[g]];
@@ -325,7 +333,6 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
   hipStream_t s = h->stream;
   HIPCHK(h, hipMemcpyAsync(h->rest64, rest.data(), 9 * S * sizeof(double), hipMemcpyHostToDevice, s));
   HIPCHK(h, hipMemcpyAsync(h->vd64, vd.data(), 3 * S * sizeof(double), hipMemcpyHostToDevice, s));
-  HIPCHK(h, hipMemcpyAsync(h->vd32, vd32.data(), 3 * S * sizeof(float), hipMemcpyHostToDevice, s));
   HIPCHK(h, hipMemcpyAsync(h->mass, mint.data(), N * sizeof(double), hipMemcpyHostToDevice, s));
   HIPCHK(h, hipMemcpyAsync(h->plastic_param, pp.data(), 3 * T * sizeof(double), hipMemcpyHostToDevice, s));
   HIPCHK(h, hipMemsetAsync(h->plastic, 0, 6 * T * sizeof(double), s));
@@ -337,11 +344,11 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
   HIPCHK(h, hipGetDeviceProperties(&prop, h->device));
   if (h->lds_elem > (size_t)prop.sharedMemPerBlock || h->lds_cg > (size_t)prop.sharedMemPerBlock)
     return fail(h, OTFEM_INVALID_ARG, "tile too large for LDS; lower target_tile_nodes");
-  int per_cu = 0;
-  HIPCHK(h, cg_occupancy(kBlock, h->lds_cg, &per_cu));
-  if (per_cu <= 0) return fail(h, OTFEM_HIP_ERROR, "CG kernel does not fit on a CU");
-  int64_t cap = (int64_t)per_cu * prop.multiProcessorCount;
-  h->cg_grid = (int)std::min<int64_t>(cap, (int64_t)L.tiles.size());
+  // material data changed: captured graphs are rebuilt on the next step
+  if (h->g_elem) (void)hipGraphExecDestroy(h->g_elem);
+  if (h->g_cg) (void)hipGraphExecDestroy(h->g_cg);
+  h->g_elem = h->g_cg = nullptr;
+  h->graph_key.clear();
   h->initialized = true;
   h->rotations_valid = false;
   return OTFEM_OK;
@@ -352,10 +359,11 @@ StepArgs make_args(otfem_solver *h, double dt, int warp, const otfem_params &P) 
   a.x = h->x; a.x0 = h->x0; a.v = h->v; a.fext = h->fext; a.mass = h->mass; a.fixed = h->fixed;
   a.rhs = h->rhs; a.r = h->r; a.p0 = h->p0; a.p1 = h->p1; a.y = h->y;
   a.tiles = h->tiles; a.halo = h->halo; a.conn = h->conn; a.slot_tet = h->slot_tet;
-  a.rest64 = h->rest64; a.vd64 = h->vd64; a.vd32 = h->vd32; a.bt = h->bt;
+  a.rest64 = h->rest64; a.vd64 = h->vd64; a.bt4 = h->bt4; a.bt8 = h->bt8;
   a.Re = h->Re; a.plastic = h->plastic; a.plastic_param = h->plastic_param;
-  a.body_tile_begin = h->body_tile_begin; a.body = h->body; a.tile_partial = h->tile_partial;
-  a.body_cnt = h->body_cnt; a.ctrl = h->ctrl;
+  a.body_tile_begin = h->body_tile_begin; a.body = h->body;
+  a.part_bb = h->part_bb; a.part_rr = h->part_rr; a.part_pq = h->part_pq;
+  a.done_count = h->done_count; a.iter_acc = h->iter_acc;
   a.S = (int64_t)h->L.slot_tet.size();
   a.T = h->L.T;
   a.ntiles = (int32_t)h->L.tiles.size();
@@ -372,22 +380,30 @@ StepArgs make_args(otfem_solver *h, double dt, int warp, const otfem_params &P) 
   a.rotation_mode = P.rotation_mode;
   a.store_rotations = P.store_rotations ? 1 : 0;
   a.plastic_on = h->plastic_on ? 1 : 0;
-  a.nblocks = h->cg_grid;
-  a.spin_limit = h->spin_limit;
+  a.store64 = (h->lc.spmv_mode == SPMV_F64_STORE64) ? 1 : 0;
   return a;
+}
+
+otfem_status resolve_timing(otfem_solver *h) {
+  for (size_t k = 0; 3 * k + 2 < h->pool_used; ++k) {
+    float e1 = 0.f, e2 = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&e1, h->pool[3 * k], h->pool[3 * k + 1]));
+    HIPCHK(h, hipEventElapsedTime(&e2, h->pool[3 * k + 1], h->pool[3 * k + 2]));
+    h->last_elem_ms = e1;
+    h->last_cg_ms = e2;
+    h->acc_elem_ms += e1;
+    h->acc_cg_ms += e2;
+    h->timed_steps++;
+  }
+  h->pool_used = 0;
+  return OTFEM_OK;
 }
 
 otfem_status sync_and_check(otfem_solver *h, otfem_stats *stats) {
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->pending = false;
-  Control c;
-  HIPCHK(h, hipMemcpy(&c, h->ctrl, sizeof(Control), hipMemcpyDeviceToHost));
-  if (c.abort) {
-    Control z{};
-    (void)hipMemcpy(h->ctrl, &z, sizeof(Control), hipMemcpyHostToDevice);
-    (void)hipMemset(h->body_cnt, 0, 2 * h->L.B * sizeof(unsigned int));
-    return fail(h, OTFEM_HIP_ERROR, "device grid barrier timed out; step aborted");
-  }
+  otfem_status ts = resolve_timing(h);
+  if (ts != OTFEM_OK) return ts;
   h->last_body.resize(h->L.B);
   HIPCHK(h, hipMemcpy(h->last_body.data(), h->body, h->L.B * sizeof(BodyState),
                       hipMemcpyDeviceToHost));
@@ -658,28 +674,60 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
     return fail(h, OTFEM_INVALID_ARG, "bad solver parameters");
   if (P.warm_start != OTFEM_WARM_PREVIOUS_VELOCITY)
     return fail(h, OTFEM_INVALID_ARG, "warm start model not loaded");
+  if (h->lc.spmv_mode == SPMV_F64_STORE64 && !h->bt8)
+    HIPCHK(h, dalloc(&h->bt8, 3 * (size_t)h->L.slot_tet.size()));
   StepArgs a = make_args(h, dt, use_stiffness_warping, P);
   hipStream_t s = h->stream;
   const bool timed = (stats != nullptr) || h->timing;
-  if (timed) HIPCHK(h, hipEventRecord(h->ev0, s));
-  HIPCHK(h, launch_element(a, kBlock, h->lds_elem, s));
-  if (timed) HIPCHK(h, hipEventRecord(h->ev1, s));
-  HIPCHK(h, launch_cg(a, h->cg_grid, kBlock, h->lds_cg, true, s));
-  if (timed) HIPCHK(h, hipEventRecord(h->ev2, s));
-  h->rotations_valid = (P.store_rotations != 0) && use_stiffness_warping;
-  if (!use_stiffness_warping) h->rotations_valid = false;
-  h->pending = true;
+  hipEvent_t e[3] = {nullptr, nullptr, nullptr};
   if (timed) {
-    HIPCHK(h, hipEventSynchronize(h->ev2));
-    float e1 = 0.f, e2 = 0.f;
-    HIPCHK(h, hipEventElapsedTime(&e1, h->ev0, h->ev1));
-    HIPCHK(h, hipEventElapsedTime(&e2, h->ev1, h->ev2));
-    h->last_elem_ms = e1;
-    h->last_cg_ms = e2;
-    h->acc_elem_ms += e1;
-    h->acc_cg_ms += e2;
-    h->timed_steps++;
+    while (h->pool.size() < h->pool_used + 3) {
+      hipEvent_t ev;
+      HIPCHK(h, hipEventCreate(&ev));
+      h->pool.push_back(ev);
+    }
+    for (int k = 0; k < 3; ++k) e[k] = h->pool[h->pool_used + k];
+    h->pool_used += 3;
+    HIPCHK(h, hipEventRecord(e[0], s));
   }
+  if (h->use_graphs) {
+    // the graphs bake in every kernel argument: rebuild when any changes
+    std::vector<unsigned char> key(sizeof(StepArgs) + sizeof(LaunchConfig));
+    std::memcpy(key.data(), &a, sizeof(StepArgs));
+    std::memcpy(key.data() + sizeof(StepArgs), &h->lc, sizeof(LaunchConfig));
+    if (key != h->graph_key || !h->g_elem || !h->g_cg) {
+      if (h->g_elem) (void)hipGraphExecDestroy(h->g_elem);
+      if (h->g_cg) (void)hipGraphExecDestroy(h->g_cg);
+      h->g_elem = h->g_cg = nullptr;
+      h->graph_key.clear();
+      hipGraph_t g;
+      HIPCHK(h, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      hipError_t ce = enqueue_element(a, h->lds_elem, s);
+      hipError_t ee = hipStreamEndCapture(s, &g);
+      if (ce != hipSuccess) return hip_fail(h, ce, "enqueue_element");
+      HIPCHK(h, ee);
+      HIPCHK(h, hipGraphInstantiate(&h->g_elem, g, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(g);
+      HIPCHK(h, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      ce = enqueue_cg(a, h->lc, h->lds_cg, s);
+      ee = hipStreamEndCapture(s, &g);
+      if (ce != hipSuccess) return hip_fail(h, ce, "enqueue_cg");
+      HIPCHK(h, ee);
+      HIPCHK(h, hipGraphInstantiate(&h->g_cg, g, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(g);
+      h->graph_key = key;
+    }
+    HIPCHK(h, hipGraphLaunch(h->g_elem, s));
+    if (timed) HIPCHK(h, hipEventRecord(e[1], s));
+    HIPCHK(h, hipGraphLaunch(h->g_cg, s));
+  } else {
+    HIPCHK(h, enqueue_element(a, h->lds_elem, s));
+    if (timed) HIPCHK(h, hipEventRecord(e[1], s));
+    HIPCHK(h, enqueue_cg(a, h->lc, h->lds_cg, s));
+  }
+  if (timed) HIPCHK(h, hipEventRecord(e[2], s));
+  h->rotations_valid = (P.store_rotations != 0) && use_stiffness_warping;
+  h->pending = true;
   if (!stats) return OTFEM_OK;
   return sync_and_check(h, stats);
 }
@@ -692,6 +740,10 @@ otfem_status otfem_synchronize(otfem_handle h, otfem_stats *stats) {
 
 otfem_status otfem_set_timing(otfem_handle h, int enable) {
   if (!h) return OTFEM_INVALID_ARG;
+  if (h->device_ready && h->pool_used) {
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->pool_used = 0;
+  }
   h->timing = enable != 0;
   h->acc_elem_ms = h->acc_cg_ms = 0.0;
   h->timed_steps = 0;
@@ -701,9 +753,39 @@ otfem_status otfem_set_timing(otfem_handle h, int enable) {
 otfem_status otfem_get_timing(otfem_handle h, double *element_ms, double *cg_ms,
                               long long *steps) {
   if (!h) return OTFEM_INVALID_ARG;
+  if (h->device_ready && h->pool_used) {
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    otfem_status ts = resolve_timing(h);
+    if (ts != OTFEM_OK) return ts;
+  }
   if (element_ms) *element_ms = h->acc_elem_ms;
   if (cg_ms) *cg_ms = h->acc_cg_ms;
   if (steps) *steps = h->timed_steps;
+  return OTFEM_OK;
+}
+
+otfem_status otfem_iteration_totals(otfem_handle h, int64_t *totals, int reset) {
+  if (!h) return OTFEM_INVALID_ARG;
+  if (!h->device_ready) return fail(h, OTFEM_STATE, "no device state yet");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (totals)
+    HIPCHK(h, hipMemcpy(totals, h->iter_acc, h->L.B * sizeof(int64_t), hipMemcpyDeviceToHost));
+  if (reset) HIPCHK(h, hipMemset(h->iter_acc, 0, h->L.B * sizeof(unsigned long long)));
+  return OTFEM_OK;
+}
+
+otfem_status otfem_set_kernel_config(otfem_handle h, int cg_block, int spmv_mode, int use_graphs) {
+  if (!h) return OTFEM_INVALID_ARG;
+  if (cg_block != 256 && cg_block != 512 && cg_block != 1024)
+    return fail(h, OTFEM_INVALID_ARG, "cg_block must be 256, 512 or 1024");
+  if (spmv_mode < SPMV_F32 || spmv_mode > SPMV_F64_STORE64)
+    return fail(h, OTFEM_INVALID_ARG, "unknown SpMV precision mode");
+  if (spmv_mode == SPMV_F64_STORE64 && !h->bt8 && h->device_ready) {
+    HIPCHK(h, dalloc(&h->bt8, 3 * (size_t)h->L.slot_tet.size()));
+  }
+  h->lc.cg_block = cg_block;
+  h->lc.spmv_mode = spmv_mode;
+  h->use_graphs = use_graphs != 0;
   return OTFEM_OK;
 }
 

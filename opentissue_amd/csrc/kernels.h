@@ -8,31 +8,21 @@
 
 namespace otfem {
 
-// Per-body CG state, written by the last tile of a body to finish a phase and
-// read by every tile of that body after the next grid barrier.
+// Per-body CG record.  Written only by the body's first tile (body_rank 0);
+// read by other workgroups in later kernels of the same step.
 struct BodyState {
   double bb;      // b.b over free dofs
   double rr0;     // r.r after the warm start
-  double d;       // r.r before the current iteration's update
-  double d2;      // p.Ap of the current iteration (clamped in reference mode)
-  double alpha;   // d / d2
-  double beta;    // d1 / d (clamped)
-  double rr;      // latest r.r
+  double rr;      // r.r at the last recorded point
+  double d;       // r.r entering the current iteration (for alpha)
   int32_t iterations;
-  int32_t active;
+  int32_t stopped;
   int32_t converged;
   int32_t flags;  // bit 0: non-finite scalar
 };
 
-// Grid-wide control words for the persistent kernel.  All start at zero and
-// return to zero at the end of a well-formed step.
-struct Control {
-  unsigned int bar_count;
-  unsigned int bar_gen;
-  unsigned int abort;          // set on a barrier time-out; every wave then exits
-  int active_bodies;
-  unsigned int pad[4];
-};
+// SpMV arithmetic inside CG (element data is FP32 in memory in every mode).
+enum SpmvMode { SPMV_F32 = 0, SPMV_F32_ACC64 = 1, SPMV_F64 = 2, SPMV_F64_STORE64 = 3 };
 
 struct StepArgs {
   // nodes, internal order, xyz interleaved
@@ -46,7 +36,7 @@ struct StepArgs {
   double *r;
   double *p0;
   double *p1;
-  float *y;
+  double *y;             // K' p of the owned nodes (consumed by the update)
   // tiles / slots
   const TileInfo *tiles;
   const int32_t *halo;
@@ -54,18 +44,20 @@ struct StepArgs {
   const int32_t *slot_tet;
   const double *rest64;  // [9][S]  grad N_k (k = 1..3) at rest
   const double *vd64;    // [3][S]  |V| * (D0, D1, D2)
-  const float *vd32;     // [3][S]
-  float *bt;             // [9][S]  R * grad N_k, refreshed each step
+  float4 *bt4;           // [3][S]  (R grad N_k, |V| D_k), refreshed each step
+  double4 *bt8;          // [3][S]  same in FP64 (only with SPMV_F64_STORE64)
   // per tet (caller order)
-  double *Re;            // [9][T] or null
-  double *plastic;       // [6][T]
-  const double *plastic_param;  // [3][T] yield, creep, max
-  // bodies / reductions
+  double *Re;            // [T][9]
+  double *plastic;       // [T][6]
+  const double *plastic_param;  // [T][3] yield, creep, max
+  // bodies / reductions (no atomics except done_count)
   const int32_t *body_tile_begin;
   BodyState *body;
-  double *tile_partial;  // [2 * ntiles]
-  unsigned int *body_cnt;  // [2 * B]
-  Control *ctrl;
+  double *part_bb;       // [ntiles]
+  double *part_rr;       // [2][ntiles]  r_k.r_k at [k & 1]
+  double *part_pq;       // [ntiles]
+  int *done_count;       // bodies whose CG stopped in this step
+  unsigned long long *iter_acc;  // [B] CG iterations summed over steps
   // sizes
   int64_t S;
   int64_t T;
@@ -84,15 +76,18 @@ struct StepArgs {
   int32_t rotation_mode;
   int32_t store_rotations;
   int32_t plastic_on;
-  int32_t nblocks;       // grid size of the persistent kernel
-  long long spin_limit;  // barrier time-out in wall-clock ticks
+  int32_t store64;       // element kernel also writes bt8
 };
 
-// Host launchers (kernels.hip).
-hipError_t launch_element(const StepArgs &a, int block, size_t lds, hipStream_t s);
-hipError_t launch_cg(const StepArgs &a, int grid, int block, size_t lds, bool cooperative,
-                     hipStream_t s);
-hipError_t cg_occupancy(int block, size_t lds, int *blocks_per_cu);
+struct LaunchConfig {
+  int cg_block = 256;   // 256, 512 or 1024 threads per tile in the CG kernels
+  int spmv_mode = SPMV_F64;
+};
+
+// One step = enqueue_element, then enqueue_cg (cg_max x {S, U} kernels and
+// the finishing kernel).
+hipError_t enqueue_element(const StepArgs &a, size_t lds_elem, hipStream_t s);
+hipError_t enqueue_cg(const StepArgs &a, const LaunchConfig &lc, size_t lds_cg, hipStream_t s);
 size_t element_lds_bytes(int max_nodes, int max_owned, int plastic_on);
 size_t cg_lds_bytes(int max_nodes, int max_owned);
 

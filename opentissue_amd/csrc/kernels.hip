@@ -1,26 +1,34 @@
 // otfem device kernels for gfx950 (MI355X).
 //
-// Two kernels make one implicit step of the corotated linear-elastic FEM of
-// OpenTissue::fem (fem_simulate.h:49-113):
+// One implicit step of the corotated linear-elastic FEM of OpenTissue::fem
+// (fem_simulate.h:49-113) is a chain of kernels on one stream (captured in a
+// hipGraph by the host):
 //
-//   k_element  one workgroup per tile, FP64 arithmetic.  Per tet copy (slot):
-//              deformation gradient F = I + sum_k (u_k - u_0) gradN_k^T from
-//              displacements, rotation R (Gram-Schmidt of F's rows as
-//              fem_update_orientation.h:191-208 / math_matrix3x3.h:535-557, or
-//              the polar factor), corotated elastic force -R Ke (R^T x - x0)
-//              (fem_stiffness_assembly.h:179-205 + fem_dynamics_assembly.h:
-//              80-91), K'v for the warm-start residual, optional plasticity
-//              (fem_add_plasticity_force.h:85-199), and the rotated gradients
-//              R gradN_k (FP32) that the SpMV uses.  Owned-node epilogue: right
-//              hand side b, initial residual r = b - A v, p = r
-//              (fem_conjugate_gradients.h:48-67), and per-body b.b, r.r.
+//   k_element      one workgroup per tile, FP64 arithmetic.  Per tet copy
+//                  (slot): deformation gradient F = I + sum_k (u_k - u_0)
+//                  gradN_k^T from displacements, rotation R (Gram-Schmidt of
+//                  F's rows as fem_update_orientation.h:191-208 /
+//                  math_matrix3x3.h:535-557, or the polar factor), corotated
+//                  elastic force -R Ke (R^T x - x0) (fem_stiffness_assembly.h:
+//                  179-205 + fem_dynamics_assembly.h:80-91), K'v for the
+//                  warm-start residual, optional plasticity
+//                  (fem_add_plasticity_force.h:85-199), and the rotated
+//                  gradients R gradN_k (FP32) that the SpMV uses.  Owned-node
+//                  epilogue: right-hand side b, r = b - A v, p = r
+//                  (fem_conjugate_gradients.h:48-67), tile partials of b.b, r.r.
 //
-//   k_cg       persistent CG (fem_conjugate_gradients.h:69-132) for every body
-//              at once, two grid barriers per iteration:
-//                phase S  p = r + beta p (owned + halo), q = A p matrix-free in
-//                         FP32 from FP64 differences, p.q        -> alpha
-//                phase U  v += alpha p, r -= alpha q, r.r          -> beta, stop
-//              then x += dt v (fem_position_update.h:25-39).
+//   k_cg_s(it)     p = r + beta p (owned + halo), q = A p matrix-free, tile
+//                  partial of p.q (fem_conjugate_gradients.h:72-99, 124-131).
+//   k_cg_u(it)     alpha = r.r / p.q, v += alpha p, r -= alpha q, tile partial
+//                  of r.r (:101-117).
+//   k_finish       records the stop of bodies that reached the cap, then
+//                  x += dt v (fem_position_update.h:25-39).
+//
+// No workgroup waits for another inside a kernel: the kernel boundary is the
+// grid barrier (cheaper on this GPU than any in-kernel barrier over hundreds
+// of workgroups).  Every workgroup sums its body's tile partials itself, in
+// tile order, so all workgroups derive identical alpha / beta / stop
+// decisions; device atomics are used only to count stopped bodies.
 //
 // Matrix-free operator.  For isotropic Ke_ij = V [D1 b_i b_j^T + D2 b_j b_i^T
 // + D2 (b_i.b_j) I] (fem_compute_ke.h:78-99 rewritten), R Ke(b) R^T equals
@@ -38,8 +46,7 @@ namespace otfem {
 
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kWaves = kBlock / 64;
+constexpr int kElemBlock = 256;
 
 __device__ __forceinline__ int xcd_remap(int bid, int nb) {
   // Workgroups are dispatched round-robin over the 8 XCDs.  Give each XCD a
@@ -57,105 +64,36 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Deterministic block sum of two doubles; the result is valid in thread 0.
-__device__ __forceinline__ void block_sum2(double &a, double &b, double *red) {
+// Deterministic block sum (fixed tree), result broadcast to every thread.
+// `red` holds BLOCK/64 + 1 doubles of LDS.
+template <int BLOCK>
+__device__ __forceinline__ double block_sum(double a, double *red) {
+  constexpr int W = BLOCK / 64;
   a = wave_sum(a);
-  b = wave_sum(b);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  if (l == 0) {
-    red[2 * w] = a;
-    red[2 * w + 1] = b;
-  }
+  if (l == 0) red[w] = a;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double s0 = 0.0, s1 = 0.0;
-    for (int i = 0; i < kWaves; ++i) {
-      s0 += red[2 * i];
-      s1 += red[2 * i + 1];
-    }
-    a = s0;
-    b = s1;
+    double s = 0.0;
+    for (int i = 0; i < W; ++i) s += red[i];
+    red[W] = s;
   }
   __syncthreads();
-}
-
-__device__ __forceinline__ double ld_agent(const double *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Grid barrier over the co-resident workgroups of the persistent kernel.
-// Bounded: a wait longer than spin_limit ticks raises ctrl->abort, and every
-// workgroup leaves at its next barrier, so the kernel always terminates.
-__device__ bool grid_barrier(Control *c, unsigned int nblocks, long long spin_limit) {
-  __shared__ int s_ok;
+  double r = red[W];
   __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned int g = __hip_atomic_load(&c->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned int arrived =
-        __hip_atomic_fetch_add(&c->bar_count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (arrived == nblocks - 1) {
-      __hip_atomic_store(&c->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&c->bar_gen, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      long long t0 = wall_clock64();
-      while (__hip_atomic_load(&c->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
-        if (__hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-        if (wall_clock64() - t0 > spin_limit) {
-          __hip_atomic_store(&c->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(2);
-      }
-    }
-    __threadfence();
-    s_ok = __hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
-  }
-  __syncthreads();
-  return s_ok != 0;
+  return r;
 }
 
-// Publishes this tile's two partial sums and tells (uniformly across the
-// workgroup) whether it is the last tile of its body to do so in `phase`.
-__device__ bool publish_partials(const StepArgs &A, int tile, int body, int phase, int ntiles_body,
-                                 double a, double b) {
-  __shared__ int s_last;
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(&A.tile_partial[2 * tile], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&A.tile_partial[2 * tile + 1], b, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned int prev = __hip_atomic_fetch_add(&A.body_cnt[2 * body + phase], 1u,
-                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    int last = (prev == (unsigned int)(ntiles_body - 1));
-    if (last) {
-      __hip_atomic_store(&A.body_cnt[2 * body + phase], 0u, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      __threadfence();
-    }
-    s_last = last;
-  }
-  __syncthreads();
-  return s_last != 0;
-}
-
-// Sum of the body's tile partials in tile order (fixed reduction tree), valid
-// in thread 0.
-__device__ void body_total(const StepArgs &A, int body, double &a, double &b, double *red) {
-  const int t0 = A.body_tile_begin[body], t1 = A.body_tile_begin[body + 1];
-  double sa = 0.0, sb = 0.0;
-  for (int t = t0 + (int)threadIdx.x; t < t1; t += kBlock) {
-    sa += ld_agent(&A.tile_partial[2 * t]);
-    sb += ld_agent(&A.tile_partial[2 * t + 1]);
-  }
-  block_sum2(sa, sb, red);
-  a = sa;
-  b = sb;
+// Sum of one body's tile partials in tile order; identical in every
+// workgroup that evaluates it.
+template <int BLOCK>
+__device__ __forceinline__ double body_sum(const double *part, int t0, int t1, double *red) {
+  double s = 0.0;
+  for (int t = t0 + (int)threadIdx.x; t < t1; t += BLOCK) s += part[t];
+  return block_sum<BLOCK>(s, red);
 }
 
 __device__ __forceinline__ bool finite(double x) { return isfinite(x); }
-
-// ---- 3x3 helpers (row-major double[9]) -----------------------------------
 
 __device__ __forceinline__ void gram_schmidt_rows(const double *F, double *R) {
   // math_matrix3x3.h:535-557 on the rows of F
@@ -250,16 +188,11 @@ __device__ __forceinline__ void matvec(const T *M, const T *x, T *y) {
   y[2] = M[6] * x[0] + M[7] * x[1] + M[8] * x[2];
 }
 
-// lds layout helpers for the element kernel
-struct ElemLds {
-  double *u, *v, *f, *kv, *fp;
-};
-
 }  // namespace
 
-__global__ __launch_bounds__(kBlock) void k_element(StepArgs A) {
+__global__ __launch_bounds__(kElemBlock) void k_element(StepArgs A) {
   extern __shared__ double lds[];
-  __shared__ double red[2 * kWaves];
+  __shared__ double red[kElemBlock / 64 + 1];
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const TileInfo ti = A.tiles[tile];
   const int nn = ti.n_owned + ti.n_halo;
@@ -270,7 +203,16 @@ __global__ __launch_bounds__(kBlock) void k_element(StepArgs A) {
   double *skv = sf + 3 * no;
   double *sfp = skv + 3 * no;
 
-  for (int l = threadIdx.x; l < nn; l += kBlock) {
+  if (ti.body_rank == 0 && threadIdx.x == 0) {
+    BodyState *w = &A.body[ti.body];
+    w->iterations = 0;
+    w->stopped = 0;
+    w->converged = 0;
+    w->flags = 0;
+  }
+  if (tile == 0 && threadIdx.x == 0) *A.done_count = 0;
+
+  for (int l = threadIdx.x; l < nn; l += kElemBlock) {
     int g = l < no ? ti.node_begin + l : A.halo[ti.halo_begin + l - no];
     for (int d = 0; d < 3; ++d) {
       su[3 * l + d] = A.x[3 * (int64_t)g + d] - A.x0[3 * (int64_t)g + d];
@@ -287,7 +229,7 @@ __global__ __launch_bounds__(kBlock) void k_element(StepArgs A) {
   __syncthreads();
 
   const int64_t S = A.S;
-  for (int s = ti.slot_begin + threadIdx.x; s < ti.slot_begin + ti.n_slots; s += kBlock) {
+  for (int s = ti.slot_begin + threadIdx.x; s < ti.slot_begin + ti.n_slots; s += kElemBlock) {
     const ushort4 c = A.conn[s];
     const int cn[4] = {c.x, c.y, c.z, c.w};
     double b[3][3];
@@ -367,11 +309,12 @@ __global__ __launch_bounds__(kBlock) void k_element(StepArgs A) {
         fe[0][d] += rt[d];
         kv[0][d] -= rw[d];
       }
-      // rotated gradient for the per-iteration SpMV
+      // rotated gradient and |V| D_k for the per-iteration SpMV
       double bt[3];
       matvec(R, b[k], bt);
-#pragma unroll
-      for (int d = 0; d < 3; ++d) A.bt[(3 * k + d) * S + s] = (float)bt[d];
+      const double vdk = k == 0 ? D0 : (k == 1 ? D1 : D2);
+      A.bt4[k * S + s] = make_float4((float)bt[0], (float)bt[1], (float)bt[2], (float)vdk);
+      if (A.store64) A.bt8[k * S + s] = make_double4(bt[0], bt[1], bt[2], vdk);
     }
 
     const int32_t st = A.slot_tet[s];
@@ -454,7 +397,7 @@ __global__ __launch_bounds__(kBlock) void k_element(StepArgs A) {
   __syncthreads();
 
   double bb = 0.0, rr = 0.0;
-  for (int l = threadIdx.x; l < no; l += kBlock) {
+  for (int l = threadIdx.x; l < no; l += kElemBlock) {
     const int64_t g = ti.node_begin + l;
     double fx[3];
     for (int d = 0; d < 3; ++d) {
@@ -484,69 +427,179 @@ __global__ __launch_bounds__(kBlock) void k_element(StepArgs A) {
       rr += rd * rd;
     }
   }
-  block_sum2(bb, rr, red);
-  const int nt_body = A.body_tile_begin[ti.body + 1] - A.body_tile_begin[ti.body];
-  if (publish_partials(A, tile, ti.body, 0, nt_body, bb, rr)) {
-    double tb, tr;
-    body_total(A, ti.body, tb, tr, red);
-    if (threadIdx.x == 0) {
-      BodyState st;
-      st.bb = tb;
-      st.rr0 = tr;
-      st.d = tr;
-      st.d2 = 0.0;
-      st.alpha = 0.0;
-      st.beta = 0.0;
-      st.rr = tr;
-      st.iterations = 0;
-      st.flags = (finite(tb) && finite(tr)) ? 0 : 1;
-      st.converged = 0;
-      st.active = 1;
-      if (st.flags) st.active = 0;
-      if (A.tol_mode == 1 && A.cg_min <= 0 && sqrt(tr) <= A.tol * sqrt(tb)) {
-        st.active = 0;
-        st.converged = 1;
-      }
-      if (A.cg_max <= 0) st.active = 0;
-      A.body[ti.body] = st;
-      __threadfence();
-      if (st.active)
-        __hip_atomic_fetch_add(&A.ctrl->active_bodies, 1, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-    }
+  bb = block_sum<kElemBlock>(bb, red);
+  rr = block_sum<kElemBlock>(rr, red);
+  if (threadIdx.x == 0) {
+    A.part_bb[tile] = bb;
+    A.part_rr[tile] = rr;  // r_0.r_0 lives at [0 & 1]
   }
 }
 
 namespace {
 
-// Reads a body's active flag and one scalar once (thread 0) and broadcasts
-// both, so every thread of the workgroup takes the same branch.
-__device__ bool body_gate(const BodyState *bs, const double *scalar, double &value) {
-  __shared__ int s_act;
-  __shared__ double s_val;
-  if (threadIdx.x == 0) {
-    s_act = __hip_atomic_load(&bs->active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s_val = ld_agent(scalar);
+// Stop test for a body entering iteration `it` (it >= 1: after the update of
+// iteration it-1 produced r.r = rr).  fem_conjugate_gradients.h:118 for the
+// reference's rule (tol_mode 0); relative residual for tol_mode 1.
+__device__ __forceinline__ bool stop_rule(const StepArgs &A, int it, double rr, double bb,
+                                          bool &converged, bool &nonfinite) {
+  converged = false;
+  nonfinite = !finite(rr) || !finite(bb);
+  if (nonfinite) return true;
+  if (it == 0) {
+    if (A.tol_mode == 1 && A.cg_min <= 0 && sqrt(rr) <= A.tol * sqrt(bb)) {
+      converged = true;
+      return true;
+    }
+    return A.cg_max <= 0;
   }
-  __syncthreads();
-  const int act = s_act;
-  value = s_val;
-  __syncthreads();
-  return act != 0;
+  if (A.tol_mode == 0) {
+    if (it - 1 >= A.cg_min && rr < A.tol) converged = true;
+  } else if (it >= A.cg_min && sqrt(rr) <= A.tol * sqrt(bb)) {
+    converged = true;
+  }
+  return converged || it >= A.cg_max;
 }
 
-__device__ void cg_phase_s(const StepArgs &A, int tile, int it, double *lds, double *red) {
+__device__ __forceinline__ void record_stop(const StepArgs &A, BodyState *w, int it, double rr,
+                                            bool converged, bool nonfinite) {
+  w->iterations = it;
+  w->rr = rr;
+  w->converged = converged ? 1 : 0;
+  w->flags |= nonfinite ? 1 : 0;
+  w->stopped = 1;
+  __threadfence();
+  atomicAdd(A.done_count, 1);
+}
+
+}  // namespace
+
+namespace {
+
+// Slot data: three packed (R grad N_k, |V| D_k) records in FP32 or FP64.
+template <typename MT, typename ST>
+__device__ __forceinline__ void load_slot(const StepArgs &A, int64_t S, int s, MT bt[3][3],
+                                          MT D[3]);
+template <>
+__device__ __forceinline__ void load_slot<float, float>(const StepArgs &A, int64_t S, int s,
+                                                        float bt[3][3], float D[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float4 q = A.bt4[k * S + s];
+    bt[k][0] = q.x; bt[k][1] = q.y; bt[k][2] = q.z; D[k] = q.w;
+  }
+}
+template <>
+__device__ __forceinline__ void load_slot<double, float>(const StepArgs &A, int64_t S, int s,
+                                                         double bt[3][3], double D[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float4 q = A.bt4[k * S + s];
+    bt[k][0] = q.x; bt[k][1] = q.y; bt[k][2] = q.z; D[k] = q.w;
+  }
+}
+template <>
+__device__ __forceinline__ void load_slot<double, double>(const StepArgs &A, int64_t S, int s,
+                                                          double bt[3][3], double D[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double4 q = A.bt8[k * S + s];
+    bt[k][0] = q.x; bt[k][1] = q.y; bt[k][2] = q.z; D[k] = q.w;
+  }
+}
+
+// y_owned += (K' p)_e for one slot: G = sum_k (p_k - p_0) bt_k^T,
+// sigma = D(G), f_k = sigma bt_k, f_0 = -(f_1 + f_2 + f_3).
+template <typename MT, typename AT>
+__device__ __forceinline__ void slot_spmv(ushort4 c, const MT bt[3][3], const MT D[3],
+                                          const double *sp, AT *sy, int no) {
+  const double p0x = sp[3 * c.x], p0y = sp[3 * c.x + 1], p0z = sp[3 * c.x + 2];
+  MT dp[3][3];
+  const int cn[3] = {c.y, c.z, c.w};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    dp[k][0] = (MT)(sp[3 * cn[k]] - p0x);
+    dp[k][1] = (MT)(sp[3 * cn[k] + 1] - p0y);
+    dp[k][2] = (MT)(sp[3 * cn[k] + 2] - p0z);
+  }
+  MT G[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      G[3 * r + q] = dp[0][r] * bt[0][q] + dp[1][r] * bt[1][q] + dp[2][r] * bt[2][q];
+  MT Sg[9];
+  stress(G, D[0], D[1], D[2], Sg);
+  MT f[4][3];
+  f[0][0] = f[0][1] = f[0][2] = (MT)0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    matvec(Sg, bt[k], f[k + 1]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) f[0][d] -= f[k + 1][d];
+  }
+  const int cc[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (cc[j] < no) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) atomicAdd(&sy[3 * cc[j] + d], (AT)f[j][d]);
+    }
+  }
+}
+
+}  // namespace
+
+// p = r + beta p; q = A p; partial p.q.   MT: element arithmetic type, AT:
+// LDS accumulation type of K'p.
+template <int BLOCK, typename MT, typename AT, typename ST>
+__global__ __launch_bounds__(BLOCK) void k_cg_s(StepArgs A, int it) {
+  extern __shared__ double lds[];
+  __shared__ double red[BLOCK / 64 + 1];
+  if (*(volatile int *)A.done_count >= A.B) return;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const TileInfo ti = A.tiles[tile];
-  const BodyState *bs = &A.body[ti.body];
-  double beta;
-  if (!body_gate(bs, &bs->beta, beta)) return;
+  BodyState *bs = &A.body[ti.body];
+  if (*(volatile int32_t *)&bs->stopped) return;
+  const int t0 = A.body_tile_begin[ti.body], t1 = A.body_tile_begin[ti.body + 1];
+  const double rr = body_sum<BLOCK>(A.part_rr + (it & 1) * A.ntiles, t0, t1, red);
+  double bb;
+  if (it == 0)
+    bb = body_sum<BLOCK>(A.part_bb, t0, t1, red);
+  else
+    bb = bs->bb;
+  bool conv, nonfin;
+  if (stop_rule(A, it, rr, bb, conv, nonfin)) {
+    if (ti.body_rank == 0 && threadIdx.x == 0) {
+      if (it == 0) {
+        bs->bb = bb;
+        bs->rr0 = rr;
+      }
+      record_stop(A, bs, it, rr, conv, nonfin);
+    }
+    return;
+  }
+  double beta = 0.0;
+  if (it > 0) {
+    double rprev = body_sum<BLOCK>(A.part_rr + ((it - 1) & 1) * A.ntiles, t0, t1, red);
+    if (fabs(rprev) < A.tiny) rprev = A.tiny;  // fem_conjugate_gradients.h:121-122
+    beta = rr / rprev;
+  }
+  if (ti.body_rank == 0 && threadIdx.x == 0) {
+    if (it == 0) {
+      bs->bb = bb;
+      bs->rr0 = rr;
+    }
+    bs->d = rr;
+    bs->iterations = it;
+  }
+
   const double *pin = (it & 1) ? A.p1 : A.p0;
   double *pout = (it & 1) ? A.p0 : A.p1;
   const int nn = ti.n_owned + ti.n_halo, no = ti.n_owned;
   double *sp = lds;
-  float *sy = reinterpret_cast<float *>(sp + 3 * nn);
+  AT *sy = reinterpret_cast<AT *>(sp + 3 * nn);
 
-  for (int l = threadIdx.x; l < nn; l += kBlock) {
+  for (int l = threadIdx.x; l < nn; l += BLOCK) {
     const int64_t g = l < no ? ti.node_begin + l : A.halo[ti.halo_begin + l - no];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -554,184 +607,121 @@ __device__ void cg_phase_s(const StepArgs &A, int tile, int it, double *lds, dou
       sp[3 * l + d] = pn;
       if (l < no) {
         pout[3 * g + d] = pn;
-        sy[3 * l + d] = 0.0f;
+        sy[3 * l + d] = (AT)0;
       }
     }
   }
   __syncthreads();
 
   const int64_t S = A.S;
-  for (int s = ti.slot_begin + threadIdx.x; s < ti.slot_begin + ti.n_slots; s += kBlock) {
-    const ushort4 c = A.conn[s];
-    float bt[3][3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-      for (int d = 0; d < 3; ++d) bt[k][d] = A.bt[(3 * k + d) * S + s];
-    const float D0 = A.vd32[s], D1 = A.vd32[S + s], D2 = A.vd32[2 * S + s];
-    const double p0x = sp[3 * c.x], p0y = sp[3 * c.x + 1], p0z = sp[3 * c.x + 2];
-    float dp[3][3];
-    const int cn[3] = {c.y, c.z, c.w};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      dp[k][0] = (float)(sp[3 * cn[k]] - p0x);
-      dp[k][1] = (float)(sp[3 * cn[k] + 1] - p0y);
-      dp[k][2] = (float)(sp[3 * cn[k] + 2] - p0z);
+  const int send = ti.slot_begin + ti.n_slots;
+  // two slots per thread per trip: both slots' loads are in flight together
+  for (int s = ti.slot_begin + threadIdx.x; s < send; s += 2 * BLOCK) {
+    const int s2 = s + BLOCK;
+    const bool two = s2 < send;
+    ushort4 c[2];
+    MT bt[2][3][3], D[2][3];
+    c[0] = A.conn[s];
+    load_slot<MT, ST>(A, S, s, bt[0], D[0]);
+    if (two) {
+      c[1] = A.conn[s2];
+      load_slot<MT, ST>(A, S, s2, bt[1], D[1]);
     }
-    float G[9];
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int q = 0; q < 3; ++q)
-        G[3 * r + q] = dp[0][r] * bt[0][q] + dp[1][r] * bt[1][q] + dp[2][r] * bt[2][q];
-    float Sg[9];
-    stress(G, D0, D1, D2, Sg);
-    float f[4][3];
-    f[0][0] = f[0][1] = f[0][2] = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      matvec(Sg, bt[k], f[k + 1]);
-#pragma unroll
-      for (int d = 0; d < 3; ++d) f[0][d] -= f[k + 1][d];
-    }
-    const int cc[4] = {c.x, c.y, c.z, c.w};
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      if (cc[j] < no) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) atomicAdd(&sy[3 * cc[j] + d], f[j][d]);
-      }
+    for (int u = 0; u < 2; ++u) {
+      if (u == 1 && !two) break;
+      slot_spmv<MT, AT>(c[u], bt[u], D[u], sp, sy, no);
     }
   }
   __syncthreads();
 
   double pq = 0.0;
-  for (int l = threadIdx.x; l < no; l += kBlock) {
+  for (int l = threadIdx.x; l < no; l += BLOCK) {
     const int64_t g = ti.node_begin + l;
     if (A.fixed[g]) continue;
     const double m = A.mass[g];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      float yd = sy[3 * l + d];
+      double yd = (double)sy[3 * l + d];
       A.y[3 * g + d] = yd;
       double p = sp[3 * l + d];
-      double q = A.cM * m * p + A.cK * (double)yd;
+      double q = A.cM * m * p + A.cK * yd;
       pq += p * q;
     }
   }
-  double unused = 0.0;
-  block_sum2(pq, unused, red);
-  const int nt_body = A.body_tile_begin[ti.body + 1] - A.body_tile_begin[ti.body];
-  if (publish_partials(A, tile, ti.body, 0, nt_body, pq, 0.0)) {
-    double d2, z;
-    body_total(A, ti.body, d2, z, red);
-    if (threadIdx.x == 0) {
-      BodyState *w = &A.body[ti.body];
-      if (fabs(d2) < A.tiny) d2 = A.tiny;  // fem_conjugate_gradients.h:101-102
-      double alpha = w->d / d2;
-      w->d2 = d2;
-      w->alpha = alpha;
-      if (!finite(alpha)) {
-        w->flags |= 1;
-        w->active = 0;
-        __threadfence();
-        __hip_atomic_fetch_add(&A.ctrl->active_bodies, -1, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
-  }
+  pq = block_sum<BLOCK>(pq, red);
+  if (threadIdx.x == 0) A.part_pq[tile] = pq;
 }
 
-__device__ void cg_phase_u(const StepArgs &A, int tile, int it, double *red) {
+// alpha = d / p.q (fem_conjugate_gradients.h:101-105); v += alpha p;
+// r -= alpha q (:110-117); partial r.r.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_cg_u(StepArgs A, int it) {
+  __shared__ double red[BLOCK / 64 + 1];
+  if (*(volatile int *)A.done_count >= A.B) return;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const TileInfo ti = A.tiles[tile];
   BodyState *bs = &A.body[ti.body];
-  double alpha;
-  if (!body_gate(bs, &bs->alpha, alpha)) return;
+  if (*(volatile int32_t *)&bs->stopped) return;
+  const int t0 = A.body_tile_begin[ti.body], t1 = A.body_tile_begin[ti.body + 1];
+  double d2 = body_sum<BLOCK>(A.part_pq, t0, t1, red);
+  if (fabs(d2) < A.tiny) d2 = A.tiny;  // fem_conjugate_gradients.h:101-102
+  const double d = bs->d;
+  const double alpha = d / d2;
+  if (!finite(alpha)) {
+    if (ti.body_rank == 0 && threadIdx.x == 0) record_stop(A, bs, it, d, false, true);
+    return;
+  }
   const double *pn = (it & 1) ? A.p0 : A.p1;
   double rr = 0.0;
-  for (int l = threadIdx.x; l < ti.n_owned; l += kBlock) {
+  for (int l = threadIdx.x; l < ti.n_owned; l += BLOCK) {
     const int64_t g = ti.node_begin + l;
     if (A.fixed[g]) continue;
     const double m = A.mass[g];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      double p = pn[3 * g + d];
-      double q = A.cM * m * p + A.cK * (double)A.y[3 * g + d];
-      A.v[3 * g + d] += alpha * p;
-      double rd = A.r[3 * g + d] - alpha * q;
-      A.r[3 * g + d] = rd;
+    for (int d3 = 0; d3 < 3; ++d3) {
+      double p = pn[3 * g + d3];
+      double q = A.cM * m * p + A.cK * A.y[3 * g + d3];
+      A.v[3 * g + d3] += alpha * p;
+      double rd = A.r[3 * g + d3] - alpha * q;
+      A.r[3 * g + d3] = rd;
       rr += rd * rd;
     }
   }
-  double unused = 0.0;
-  block_sum2(rr, unused, red);
-  const int nt_body = A.body_tile_begin[ti.body + 1] - A.body_tile_begin[ti.body];
-  if (publish_partials(A, tile, ti.body, 1, nt_body, rr, 0.0)) {
-    double d1, z;
-    body_total(A, ti.body, d1, z, red);
-    if (threadIdx.x == 0) {
-      BodyState *w = bs;
-      const int it1 = w->iterations + 1;
-      w->iterations = it1;
-      w->rr = d1;
-      bool stop = false;
-      if (!finite(d1)) {
-        w->flags |= 1;
-        stop = true;
-      } else if (A.tol_mode == 0) {
-        // fem_conjugate_gradients.h:118 (iteration index is it1 - 1)
-        if (it1 - 1 >= A.cg_min && d1 < A.tol) {
-          w->converged = 1;
-          stop = true;
-        }
-      } else if (it1 >= A.cg_min && sqrt(d1) <= A.tol * sqrt(w->bb)) {
-        w->converged = 1;
-        stop = true;
-      }
-      if (it1 >= A.cg_max) stop = true;
-      if (!stop) {
-        double d = w->d;
-        if (fabs(d) < A.tiny) d = A.tiny;  // fem_conjugate_gradients.h:121-122
-        w->beta = d1 / d;
-        w->d = d1;
-      } else {
-        w->active = 0;
-        __threadfence();
-        __hip_atomic_fetch_add(&A.ctrl->active_bodies, -1, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
-  }
+  rr = block_sum<BLOCK>(rr, red);
+  if (threadIdx.x == 0) A.part_rr[((it + 1) & 1) * A.ntiles + tile] = rr;
 }
 
-}  // namespace
-
-__global__ __launch_bounds__(kBlock) void k_cg(StepArgs A) {
-  extern __shared__ double lds[];
-  __shared__ double red[2 * kWaves];
-  __shared__ int s_active;
-  const int lb = xcd_remap(blockIdx.x, gridDim.x);
-  const int nb = gridDim.x;
-  for (int it = 0; it < A.cg_max; ++it) {
-    if (threadIdx.x == 0)
-      s_active = __hip_atomic_load(&A.ctrl->active_bodies, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    if (s_active <= 0) break;
-    for (int t = lb; t < A.ntiles; t += nb) cg_phase_s(A, t, it, lds, red);
-    if (!grid_barrier(A.ctrl, nb, A.spin_limit)) return;
-    for (int t = lb; t < A.ntiles; t += nb) cg_phase_u(A, t, it, red);
-    if (!grid_barrier(A.ctrl, nb, A.spin_limit)) return;
-  }
-  // fem_position_update.h:33-38
-  for (int t = lb; t < A.ntiles; t += nb) {
-    const TileInfo ti = A.tiles[t];
-    for (int l = threadIdx.x; l < ti.n_owned; l += kBlock) {
-      const int64_t g = ti.node_begin + l;
-      if (A.fixed[g]) continue;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) A.x[3 * g + d] += A.dt * A.v[3 * g + d];
+// Bodies still running after cg_max iterations stop here; then x += dt v.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_finish(StepArgs A) {
+  __shared__ double red[BLOCK / 64 + 1];
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const TileInfo ti = A.tiles[tile];
+  BodyState *bs = &A.body[ti.body];
+  if (ti.body_rank == 0) {
+    const int stopped = *(volatile int32_t *)&bs->stopped;
+    if (!stopped) {
+      const int t0 = A.body_tile_begin[ti.body], t1 = A.body_tile_begin[ti.body + 1];
+      const int it = A.cg_max;
+      const double rr = body_sum<BLOCK>(A.part_rr + (it & 1) * A.ntiles, t0, t1, red);
+      if (threadIdx.x == 0) {
+        bool conv, nonfin;
+        stop_rule(A, it, rr, bs->bb, conv, nonfin);
+        bs->iterations = it;
+        bs->rr = rr;
+        bs->converged = conv ? 1 : 0;
+        bs->flags |= nonfin ? 1 : 0;
+        bs->stopped = 1;
+      }
     }
+    if (threadIdx.x == 0) A.iter_acc[ti.body] += (unsigned long long)bs->iterations;
+  }
+  for (int l = threadIdx.x; l < ti.n_owned; l += BLOCK) {
+    const int64_t g = ti.node_begin + l;
+    if (A.fixed[g]) continue;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) A.x[3 * g + d] += A.dt * A.v[3 * g + d];
   }
 }
 
@@ -740,30 +730,47 @@ size_t element_lds_bytes(int max_nodes, int max_owned, int plastic_on) {
 }
 
 size_t cg_lds_bytes(int max_nodes, int max_owned) {
-  return sizeof(double) * 3 * (size_t)max_nodes + sizeof(float) * 3 * (size_t)max_owned;
+  // FP64 p for all tile nodes + K'p accumulators (FP64 covers every mode)
+  return sizeof(double) * 3 * (size_t)max_nodes + sizeof(double) * 3 * (size_t)max_owned;
 }
 
-hipError_t launch_element(const StepArgs &a, int block, size_t lds, hipStream_t s) {
-  if (block != kBlock) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_element, dim3(a.ntiles), dim3(kBlock), lds, s, a);
-  return hipGetLastError();
-}
+namespace {
 
-hipError_t launch_cg(const StepArgs &a, int grid, int block, size_t lds, bool cooperative,
-                     hipStream_t s) {
-  if (block != kBlock) return hipErrorInvalidValue;
-  if (cooperative) {
-    StepArgs copy = a;
-    void *args[] = {&copy};
-    return hipLaunchCooperativeKernel(reinterpret_cast<void *>(k_cg), dim3(grid), dim3(kBlock),
-                                      args, (unsigned int)lds, s);
+template <int BLOCK, typename MT, typename AT, typename ST>
+hipError_t enqueue_iterations(const StepArgs &a, size_t lds_cg, hipStream_t s) {
+  for (int it = 0; it < a.cg_max; ++it) {
+    hipLaunchKernelGGL((k_cg_s<BLOCK, MT, AT, ST>), dim3(a.ntiles), dim3(BLOCK), lds_cg, s, a, it);
+    hipLaunchKernelGGL((k_cg_u<BLOCK>), dim3(a.ntiles), dim3(BLOCK), 0, s, a, it);
   }
-  hipLaunchKernelGGL(k_cg, dim3(grid), dim3(kBlock), lds, s, a);
+  hipLaunchKernelGGL((k_finish<BLOCK>), dim3(a.ntiles), dim3(BLOCK), 0, s, a);
   return hipGetLastError();
 }
 
-hipError_t cg_occupancy(int block, size_t lds, int *blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_cg, block, lds);
+template <int BLOCK>
+hipError_t enqueue_mode(const StepArgs &a, int mode, size_t lds_cg, hipStream_t s) {
+  switch (mode) {
+    case SPMV_F32: return enqueue_iterations<BLOCK, float, float, float>(a, lds_cg, s);
+    case SPMV_F32_ACC64: return enqueue_iterations<BLOCK, float, double, float>(a, lds_cg, s);
+    case SPMV_F64: return enqueue_iterations<BLOCK, double, double, float>(a, lds_cg, s);
+    case SPMV_F64_STORE64: return enqueue_iterations<BLOCK, double, double, double>(a, lds_cg, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+hipError_t enqueue_element(const StepArgs &a, size_t lds_elem, hipStream_t s) {
+  hipLaunchKernelGGL(k_element, dim3(a.ntiles), dim3(kElemBlock), lds_elem, s, a);
+  return hipGetLastError();
+}
+
+hipError_t enqueue_cg(const StepArgs &a, const LaunchConfig &lc, size_t lds_cg, hipStream_t s) {
+  switch (lc.cg_block) {
+    case 256: return enqueue_mode<256>(a, lc.spmv_mode, lds_cg, s);
+    case 512: return enqueue_mode<512>(a, lc.spmv_mode, lds_cg, s);
+    case 1024: return enqueue_mode<1024>(a, lc.spmv_mode, lds_cg, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace otfem

@@ -188,6 +188,54 @@ std::string build_layout(int64_t N, int64_t T, const double *coords, const int32
         L.slot_conn[4 * s + c] = (uint16_t)li;
       }
     ti.n_halo = (int32_t)(L.halo.size() - ti.halo_begin);
+    // Order the tile's slots in windows of 64 (one wave instruction) in which
+    // no two slots share an owned corner, so the per-corner LDS atomics of a
+    // wave never hit the same address.  Windows are filled in turn with the
+    // first remaining compatible slots; a window that cannot be completed is
+    // topped up with the next remaining slots so later windows stay aligned.
+    {
+      const int32_t n = ti.n_slots, s0 = ti.slot_begin;
+      std::vector<int32_t> order;
+      order.reserve(n);
+      std::vector<char> taken(n, 0);
+      std::vector<int32_t> mark(ti.n_owned, -1);
+      int32_t first = 0, window = 0;
+      while ((int32_t)order.size() < n) {
+        int32_t got = 0;
+        for (int32_t k = first; k < n && got < 64; ++k) {
+          if (taken[k]) continue;
+          bool ok = true;
+          for (int c = 0; c < 4 && ok; ++c) {
+            int32_t li = L.slot_conn[4 * (s0 + k) + c];
+            if (li < ti.n_owned && mark[li] == window) ok = false;
+          }
+          if (!ok) continue;
+          for (int c = 0; c < 4; ++c) {
+            int32_t li = L.slot_conn[4 * (s0 + k) + c];
+            if (li < ti.n_owned) mark[li] = window;
+          }
+          taken[k] = 1;
+          order.push_back(k);
+          ++got;
+        }
+        for (int32_t k = first; k < n && got < 64; ++k) {
+          if (taken[k]) continue;
+          taken[k] = 1;
+          order.push_back(k);
+          ++got;
+        }
+        while (first < n && taken[first]) ++first;
+        ++window;
+      }
+      std::vector<uint16_t> conn(4 * (size_t)n);
+      std::vector<int32_t> tet(n);
+      for (int32_t k = 0; k < n; ++k) {
+        for (int c = 0; c < 4; ++c) conn[4 * k + c] = L.slot_conn[4 * (s0 + order[k]) + c];
+        tet[k] = L.slot_tet[s0 + order[k]];
+      }
+      std::copy(conn.begin(), conn.end(), L.slot_conn.begin() + 4 * (int64_t)s0);
+      std::copy(tet.begin(), tet.end(), L.slot_tet.begin() + s0);
+    }
     L.max_tile_nodes = std::max(L.max_tile_nodes, ti.n_owned + ti.n_halo);
     L.max_tile_owned = std::max(L.max_tile_owned, ti.n_owned);
     L.max_tile_slots = std::max(L.max_tile_slots, ti.n_slots);

@@ -107,6 +107,8 @@ SIGNATURES = {
     "otfem_synchronize": (C.c_int, [_H, C.POINTER(Stats)]),
     "otfem_set_timing": (C.c_int, [_H, C.c_int]),
     "otfem_get_timing": (C.c_int, [_H, _dp, _dp, C.POINTER(C.c_longlong)]),
+    "otfem_iteration_totals": (C.c_int, [_H, C.POINTER(C.c_int64), C.c_int]),
+    "otfem_set_kernel_config": (C.c_int, [_H, C.c_int, C.c_int, C.c_int]),
     "otfem_body_stats": (C.c_int, [_H, C.POINTER(C.c_int), _dp]),
     "otfem_update_original_coord": (C.c_int, [_H]),
     "otfem_export_K": (C.c_int, [_H, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
@@ -291,6 +293,15 @@ class Solver:
         n = C.c_longlong()
         self._check(self.lib.otfem_get_timing(self.h, C.byref(e), C.byref(c), C.byref(n)))
         return e.value, c.value, n.value
+
+    def iteration_totals(self, reset=False):
+        tot = np.empty(self.B, dtype=np.int64)
+        self._check(self.lib.otfem_iteration_totals(self.h, tot.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                    1 if reset else 0))
+        return tot
+
+    def set_kernel_config(self, cg_block=256, spmv_mode=1, use_graphs=True):
+        self._check(self.lib.otfem_set_kernel_config(self.h, cg_block, spmv_mode, 1 if use_graphs else 0))
 
     def body_stats(self):
         it = np.empty(self.B, dtype=np.int32)

@@ -1,0 +1,112 @@
+"""Concrete, seeded workloads for the FEM path.
+
+Each builder returns a :class:`Workload` with the mesh (possibly a
+CSR-of-bodies batch), fixed mask, per-tet materials, external forces and the
+stepping parameters.  Unspecified quantities default to the single-lattice
+case's values: edge h = 0.01 m, rho = 1000 kg/m^3, Rayleigh alpha = 0.1,
+beta = 0.001, traction direction -z.
+
+* ``cantilever``  10x10x40 cubes (I=10, J=40, K=10 so the long axis is world
+  z), E 5e5, nu 0.33, gravity, nodes at z = 0 fixed, dt 0.01.
+* ``cube64``      64^3 cubes, interior jitter +/-0.2 h, E log-uniform per tet
+  in [1e5, 1e6], nu 0.3, face x = 0 fixed, 1e3 Pa traction on x = max,
+  dt 0.005.
+* ``bodies``      many jittered lattice bodies with per-axis dims uniform in
+  [lo, hi] cubes, per-body E log-uniform [1e4, 1e6] and nu uniform
+  [0.25, 0.45], fixed face + traction per body, dt 0.01.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import mesh as M
+
+
+@dataclass
+class Workload:
+    name: str
+    coords: np.ndarray
+    tets: np.ndarray
+    fixed: np.ndarray
+    young: np.ndarray
+    poisson: np.ndarray
+    density: np.ndarray
+    forces: np.ndarray
+    dt: float
+    steps: int
+    body_node_offsets: np.ndarray = None
+    body_tet_offsets: np.ndarray = None
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def num_nodes(self):
+        return self.coords.shape[0]
+
+    @property
+    def num_tets(self):
+        return self.tets.shape[0]
+
+    @property
+    def num_bodies(self):
+        return 1 if self.body_node_offsets is None else len(self.body_node_offsets) - 1
+
+
+def cantilever(I=10, J=40, K=10, h=0.01, E=5e5, nu=0.33, rho=1000.0, g=-9.81):
+    c, t = M.generate_blocks(I, J, K, h, h, h)
+    fixed = (np.abs(c[:, 2]) < 1e-12 * max(h, 1.0)).astype(np.uint8)
+    T = t.shape[0]
+    mass = M.lumped_mass(c, t, rho)
+    f = M.gravity_forces(np.where(fixed == 1, 0.0, mass), fixed, (0.0, 0.0, g))
+    return Workload("cantilever_%dx%dx%d" % (I, J, K), c, t, fixed, np.full(T, E), np.full(T, nu),
+                    np.full(T, rho), f, 0.01, 100, meta=dict(h=h, gravity=g))
+
+
+def cube64(n=64, h=0.01, jitter=0.2, seed=0, traction=1e3, direction=(0.0, 0.0, -1.0),
+           rho=1000.0, nu=0.3, E_range=(1e5, 1e6)):
+    c0, t = M.generate_blocks(n, n, n, h, h, h)
+    c = M.jitter_interior(c0, t, h, jitter, seed) if jitter > 0 else c0
+    rng = np.random.default_rng(seed + 1)
+    T = t.shape[0]
+    E = np.exp(rng.uniform(np.log(E_range[0]), np.log(E_range[1]), size=T))
+    fixed = M.face_nodes(c, 0, 0).astype(np.uint8)
+    f = M.traction_forces(c, t, 0, 1, traction * np.asarray(direction, dtype=np.float64))
+    f[fixed == 1] = 0.0
+    return Workload("cube%d_jitter%.2f" % (n, jitter), c, t, fixed, E, np.full(T, nu),
+                    np.full(T, rho), f, 0.005, 200,
+                    meta=dict(h=h, seed=seed, traction=traction, direction=list(direction)))
+
+
+def bodies(count=512, lo=4, hi=24, h=0.01, jitter=0.2, seed=0, traction=1e3, rho=1000.0,
+           random_traction=False, traction_range=(0.5e3, 2e3)):
+    rng = np.random.default_rng(seed)
+    dims = rng.integers(lo, hi + 1, size=(count, 3))
+    Es = np.exp(rng.uniform(np.log(1e4), np.log(1e6), size=count))
+    nus = rng.uniform(0.25, 0.45, size=count)
+    parts, fixed_l, E_l, nu_l, f_l = [], [], [], [], []
+    for b in range(count):
+        I, J, K = (int(v) for v in dims[b])
+        c0, t = M.generate_blocks(I, J, K, h, h, h)
+        c = M.jitter_interior(c0, t, h, jitter, seed * 100003 + b) if jitter > 0 else c0
+        fx = M.face_nodes(c, 0, 0).astype(np.uint8)
+        if random_traction:
+            mag = rng.uniform(*traction_range)
+            d = rng.normal(size=3)
+            d /= np.linalg.norm(d)
+            tr = mag * d
+        else:
+            tr = traction * np.array([0.0, 0.0, -1.0])
+        f = M.traction_forces(c, t, 0, 1, tr)
+        f[fx == 1] = 0.0
+        parts.append((c, t))
+        fixed_l.append(fx)
+        E_l.append(np.full(t.shape[0], Es[b]))
+        nu_l.append(np.full(t.shape[0], nus[b]))
+        f_l.append(f)
+    coords, tets, no, to = M.pack_bodies(parts)
+    T = tets.shape[0]
+    return Workload("bodies%d_dims%d-%d" % (count, lo, hi), coords, tets,
+                    np.concatenate(fixed_l), np.concatenate(E_l), np.concatenate(nu_l),
+                    np.full(T, rho), np.concatenate(f_l, 0), 0.01, 100, no, to,
+                    meta=dict(h=h, seed=seed, dims=dims.tolist()))

@@ -1,0 +1,117 @@
+"""C ABI checks that need no GPU: the library loads, exports every function
+include/otfem/otfem.h declares, validates its inputs and builds the tile
+layout on the host."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "otfem", "otfem.h")
+
+
+@pytest.fixture(scope="module")
+def fem():
+    lib = os.path.join(ROOT, "opentissue_amd", "libotfem.so")
+    if not os.path.exists(lib):
+        subprocess.run(["make", "-j4", "-C", os.path.join(ROOT, "opentissue_amd", "csrc")], check=True)
+    from opentissue_amd import fem as F
+    F.load_library()
+    return F
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(otfem_[A-Za-z_0-9]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported(fem):
+    names = declared_functions()
+    assert len(names) >= 25
+    lib = C.CDLL(fem.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the Python mirror binds each of them
+    assert sorted(fem.SIGNATURES) == names
+
+
+def test_reference_defaults(fem):
+    p = fem.reference_params()
+    # fem_simulate.h:104,108-109 and fem_conjugate_gradients.h:43-44
+    assert (p.alpha, p.beta, p.cg_min_iterations, p.cg_max_iterations) == (2.0, 0.0, 20, 20)
+    assert (p.tol_mode, p.tol, p.tiny, p.rotation_mode, p.warm_start) == (0, 1e-3, 1e-10, 0, 0)
+    q = fem.default_params(1)
+    assert (q.alpha, q.beta, q.tol, q.cg_max_iterations, q.tol_mode, q.tiny) == (0.1, 0.001, 1e-6, 200, 1, 0.0)
+    assert fem.load_library().otfem_default_params(C.byref(p), 7) == fem.INVALID_ARG
+
+
+def test_create_validates_input(fem):
+    from opentissue_amd import mesh
+    c, t = mesh.generate_blocks(2, 2, 2, 0.1, 0.1, 0.1)
+    bad = t.copy()
+    bad[3, 2] = len(c)
+    with pytest.raises(fem.OtfemError):
+        fem.Solver(c, bad)
+    bad[3, 2] = -1
+    with pytest.raises(fem.OtfemError):
+        fem.Solver(c, bad)
+    # body offsets must cover the mesh and tets must stay inside their body
+    c2, t2, no, to = mesh.pack_bodies([(c, t), (c, t)])
+    fem.Solver(c2, t2, body_node_offsets=no, body_tet_offsets=to).close()
+    short = no.copy()
+    short[-1] -= 1  # offsets must end at num_nodes
+    with pytest.raises(fem.OtfemError):
+        fem.Solver(c2, t2, body_node_offsets=short, body_tet_offsets=to)
+    cross = t2.copy()
+    cross[0, 0] = no[1]  # body 0 tet pointing into body 1
+    with pytest.raises(fem.OtfemError):
+        fem.Solver(c2, cross, body_node_offsets=no, body_tet_offsets=to)
+
+
+@pytest.mark.parametrize("dims,tile", [((10, 40, 10), 0), ((6, 7, 5), 64), ((1, 1, 1), 0)])
+def test_layout_counts(fem, dims, tile):
+    from opentissue_amd import mesh
+    c, t = mesh.generate_blocks(*dims, 0.01, 0.01, 0.01)
+    s = fem.Solver(c, t, target_tile_nodes=tile)
+    info = s.layout_info()
+    N, T = len(c), len(t)
+    want = max(1, -(-N // (tile or 512)))
+    assert info["tiles"] == want
+    # every tet has 1..4 copies (one per distinct owning tile of its corners)
+    assert T <= info["slots"] <= 4 * T
+    assert info["max_tile_nodes"] <= N
+    if info["tiles"] == 1:
+        assert info["slots"] == T and info["halo"] == 0
+    s.close()
+
+
+def test_batched_layout_never_mixes_bodies(fem):
+    from opentissue_amd import mesh
+    bodies = [mesh.generate_blocks(a, b, c, 0.01, 0.01, 0.01) for a, b, c in [(1, 1, 1), (3, 2, 4), (2, 2, 2)]]
+    c, t, no, to = mesh.pack_bodies(bodies)
+    s = fem.Solver(c, t, body_node_offsets=no, body_tet_offsets=to, target_tile_nodes=16)
+    info = s.layout_info()
+    # tiles per body = ceil(N_b / 16), summed
+    assert info["tiles"] == sum(-(-(no[b + 1] - no[b]) // 16) for b in range(3))
+    s.close()
+
+
+def test_calls_before_init_report_state(fem):
+    from opentissue_amd import mesh
+    c, t = mesh.generate_blocks(1, 1, 1, 0.1, 0.1, 0.1)
+    s = fem.Solver(c, t)
+    st = fem.Stats()
+    rc = s.lib.otfem_step(s.h, 0.01, 1, None, C.byref(st))
+    assert rc == fem.STATE
+    m = np.empty(len(c))
+    assert s.lib.otfem_get_mass(s.h, m.ctypes.data_as(C.POINTER(C.c_double))) == fem.STATE
+    n = C.c_int()
+    fem.load_library().otfem_device_count(C.byref(n))
+    if n.value == 0:
+        assert s.lib.otfem_init(s.h, 5e5, 0.3, 1000.0, 0, 0, 0) == fem.NO_DEVICE
+    s.close()

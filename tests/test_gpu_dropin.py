@@ -1,0 +1,53 @@
+"""The C++ drop-in header (<otfem/fem.hpp>) driven by caller code written
+against the reference's fem API, compared step by step with the FP64 oracle
+(each oracle step starts from the state the C++ program recorded)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, rel_disp_error
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(ROOT, "tests", "cpp", "build", "dropin_c1")
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_cpp_dropin_matches_oracle_per_step(tmp_path, mode):
+    import oracle
+    from opentissue_amd import mesh
+    assert os.path.exists(EXE), "build() compiles tests/cpp/dropin_c1.cpp"
+    I, J, K, steps = 4, 8, 3, 5
+    out = tmp_path / "x.bin"
+    r = subprocess.run([EXE, str(I), str(J), str(K), str(steps), str(mode), str(out)],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    c, t = mesh.generate_blocks(I, J, K, 0.01, 0.01, 0.01)
+    N = len(c)
+    data = np.fromfile(out, dtype=np.float64)
+    states = data[:(steps + 1) * 6 * N].reshape(steps + 1, 2, N, 3)
+    nk = int(data[(steps + 1) * 6 * N])
+    fixed = (c[:, 2] == 0).astype(np.uint8)
+    o = oracle.Oracle(c, t, fixed)
+    o.init(5e5, 0.33, 1000.0)
+    m = o.mass()
+    o.set_external_forces(mesh.gravity_forces(np.where(fixed == 1, 0.0, m), fixed))
+    P = oracle.reference_params() if mode == 0 else oracle.config_params()
+    np.testing.assert_array_equal(states[0, 0], c)
+    errs = []
+    for k in range(steps):
+        o.set_state(states[k, 0], states[k, 1])
+        o.simulate(0.01, True, P)
+        xo, _ = o.state()
+        errs.append(rel_disp_error(states[k + 1, 0], xo, c))
+    print("per-step errors", errs)
+    assert max(errs) < 1e-6
+    # fem::export_K: an identity entry per fixed dof plus 9 per free-free block
+    Kd = o.export_K_dense()
+    fr = np.repeat(fixed == 0, 3)
+    Kff = Kd[np.ix_(fr, fr)]
+    blocks = (np.abs(Kff.reshape(fr.sum() // 3, 3, fr.sum() // 3, 3)).sum((1, 3)) > 0).sum()
+    assert nk == 9 * blocks + 3 * int(fixed.sum())
