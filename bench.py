@@ -9,8 +9,8 @@ A step is one full implicit step (rotations + elastic forces + right-hand
 side + CG + position update) over the whole mesh, inputs resident in HBM.
 
 Output: one JSON line.  `value` is tet-steps/s over exactly --steps timed
-steps from rest; `roofline` prices the dominant kernel (the persistent CG
-kernel) by its algorithmic bytes (100 B/tet + 195 B/node per CG iteration,
+steps from rest; `roofline` prices the dominant kernels (one k_cg_iter launch
+per CG iteration, plus k_finish) by their algorithmic bytes (100 B/tet + 195 B/node per CG iteration,
 FP64 v and r, plus 36 B/node for the position update) against the MI355X HBM
 peak; `cpu_baseline` times the FP64 CPU restatement of the reference step
 (one thread) on a bounded sample of the same workload.
@@ -128,6 +128,8 @@ def main():
     params_kw = dict(alpha=0.1, beta=0.001, tol=1e-6, cap=200)
     P = fem.config_params(**params_kw)
     layout = s.layout_info()
+    best_block = s.autotune(w.dt, True, P, steps=2)
+    log(f"autotuned CG block size: {best_block} threads per tile ({layout['tiles']} tiles)")
 
     # warmup, then back to rest so the timed steps are steps 1..K from rest
     for _ in range(args.warmup):
@@ -163,7 +165,7 @@ def main():
         f"CG iterations {total_iters} ({total_iters / max(1, args.steps) / w.num_bodies:.1f} per step per body)")
 
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"k_cg_{args.config}_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", f"k_cg_iter_{args.config}_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -185,7 +187,7 @@ def main():
             "workload": f"{w.name}: {w.num_bodies} body/bodies, {T} tets, {N} nodes, "
                         f"dt {w.dt}, CG rel-res 1e-6 cap 200, alpha 0.1 beta 0.001, warm start v^i",
             "tets": T, "nodes": N, "bodies": w.num_bodies,
-            "tiles": layout["tiles"], "slots": layout["slots"],
+            "tiles": layout["tiles"], "slots": layout["slots"], "cg_block": best_block,
         },
         "cg_iterations_per_step": total_iters / args.steps / w.num_bodies,
         "kernel_ms_per_step": {"element": elem_ms / max(1, nsteps), "cg": cg_ms / max(1, nsteps)},
@@ -196,7 +198,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_cg (persistent CG + position update)",
+            "kernel": "CG kernels (k_cg_iter x iterations + k_finish)",
             "algorithmic_bytes_per_launch": cg_bytes / args.steps,
             "avg_launch_ms": cg_ms / max(1, nsteps),
         },

@@ -157,6 +157,12 @@ otfem_status otfem_iteration_totals(otfem_handle h, int64_t *totals, int reset);
  * counterpart (tuning of this implementation). */
 enum { OTFEM_SPMV_F32 = 0, OTFEM_SPMV_F32_ACC64 = 1, OTFEM_SPMV_F64 = 2, OTFEM_SPMV_F64_STORE64 = 3 };
 otfem_status otfem_set_kernel_config(otfem_handle h, int cg_block, int spmv_mode, int use_graphs);
+/* Autotuner: times `steps` steps with each CG block size (256/512/1024
+ * threads per tile) on this mesh and these settings, restores the state
+ * (x, v, f_ext, plastic strain) after every trial, and keeps the fastest.
+ * No reference counterpart (tuning of this implementation). */
+otfem_status otfem_autotune(otfem_handle h, double dt, int use_stiffness_warping,
+                            const otfem_params *params, int steps, int *best_block);
 /* Per-body CG iterations and final relative residual of the last step. */
 otfem_status otfem_body_stats(otfem_handle h, int *iterations, double *rel_residual);
 

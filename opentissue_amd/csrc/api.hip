@@ -40,8 +40,7 @@ struct otfem_solver {
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   // device buffers
   double *x = nullptr, *x0 = nullptr, *v = nullptr, *fext = nullptr, *mass = nullptr;
-  double *rhs = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr;
-  double *y = nullptr;
+  double *rhs = nullptr, *r = nullptr, *p = nullptr, *y = nullptr;
   uint8_t *fixed = nullptr;
   TileInfo *tiles = nullptr;
   int32_t *halo = nullptr;
@@ -53,7 +52,7 @@ struct otfem_solver {
   double *Re = nullptr, *plastic = nullptr, *plastic_param = nullptr;
   int32_t *body_tile_begin = nullptr;
   BodyState *body = nullptr;
-  double *part_bb = nullptr, *part_rr = nullptr, *part_pq = nullptr;
+  double *part_bb = nullptr, *part_rr = nullptr, *part4 = nullptr;
   int *done_count = nullptr;
   unsigned long long *iter_acc = nullptr;
   size_t lds_elem = 0, lds_cg = 0;
@@ -104,12 +103,12 @@ void dfree(T *&p) {
 
 void free_device(otfem_solver *h) {
   dfree(h->x); dfree(h->x0); dfree(h->v); dfree(h->fext); dfree(h->mass);
-  dfree(h->rhs); dfree(h->r); dfree(h->p0); dfree(h->p1); dfree(h->y); dfree(h->fixed);
+  dfree(h->rhs); dfree(h->r); dfree(h->p); dfree(h->y); dfree(h->fixed);
   dfree(h->tiles); dfree(h->halo); dfree(h->conn); dfree(h->slot_tet);
   dfree(h->rest64); dfree(h->vd64); dfree(h->bt4); dfree(h->bt8);
   dfree(h->Re); dfree(h->plastic); dfree(h->plastic_param);
   dfree(h->body_tile_begin); dfree(h->body); dfree(h->part_bb); dfree(h->part_rr);
-  dfree(h->part_pq); dfree(h->done_count); dfree(h->iter_acc);
+  dfree(h->part4); dfree(h->done_count); dfree(h->iter_acc);
   if (h->g_elem) (void)hipGraphExecDestroy(h->g_elem);
   if (h->g_cg) (void)hipGraphExecDestroy(h->g_cg);
   h->g_elem = h->g_cg = nullptr;
@@ -172,10 +171,9 @@ otfem_status ensure_device(otfem_solver *h) {
   HIPCHK(h, dalloc(&h->fext, 3 * N));
   HIPCHK(h, dalloc(&h->mass, N));
   HIPCHK(h, dalloc(&h->rhs, 3 * N));
-  HIPCHK(h, dalloc(&h->r, 3 * N));
-  HIPCHK(h, dalloc(&h->p0, 3 * N));
-  HIPCHK(h, dalloc(&h->p1, 3 * N));
-  HIPCHK(h, dalloc(&h->y, 3 * N));
+  HIPCHK(h, dalloc(&h->r, 6 * N));
+  HIPCHK(h, dalloc(&h->p, 6 * N));
+  HIPCHK(h, dalloc(&h->y, 6 * N));
   HIPCHK(h, dalloc(&h->fixed, N));
   HIPCHK(h, dalloc(&h->tiles, nt));
   HIPCHK(h, dalloc(&h->halo, L.halo.size()));
@@ -190,8 +188,8 @@ otfem_status ensure_device(otfem_solver *h) {
   HIPCHK(h, dalloc(&h->body_tile_begin, B + 1));
   HIPCHK(h, dalloc(&h->body, B));
   HIPCHK(h, dalloc(&h->part_bb, nt));
-  HIPCHK(h, dalloc(&h->part_rr, 2 * nt));
-  HIPCHK(h, dalloc(&h->part_pq, nt));
+  HIPCHK(h, dalloc(&h->part_rr, nt));
+  HIPCHK(h, dalloc(&h->part4, 8 * nt));
   HIPCHK(h, dalloc(&h->done_count, 1));
   HIPCHK(h, dalloc(&h->iter_acc, B));
   hipStream_t s = h->stream;
@@ -217,14 +215,14 @@ otfem_status ensure_device(otfem_solver *h) {
   for (int64_t g = 0; g < N; ++g) fx[g] = h->fixed_host[L.node_of_internal[g]];
   HIPCHK(h, hipMemcpyAsync(h->fixed, fx.data(), N, hipMemcpyHostToDevice, s));
   HIPCHK(h, hipMemsetAsync(h->fext, 0, 3 * N * sizeof(double), s));
-  HIPCHK(h, hipMemsetAsync(h->p0, 0, 3 * N * sizeof(double), s));
-  HIPCHK(h, hipMemsetAsync(h->p1, 0, 3 * N * sizeof(double), s));
-  HIPCHK(h, hipMemsetAsync(h->y, 0, 3 * N * sizeof(double), s));
+  HIPCHK(h, hipMemsetAsync(h->r, 0, 6 * N * sizeof(double), s));
+  HIPCHK(h, hipMemsetAsync(h->p, 0, 6 * N * sizeof(double), s));
+  HIPCHK(h, hipMemsetAsync(h->y, 0, 6 * N * sizeof(double), s));
   HIPCHK(h, hipMemsetAsync(h->plastic, 0, 6 * T * sizeof(double), s));
   HIPCHK(h, hipMemsetAsync(h->body, 0, B * sizeof(BodyState), s));
   HIPCHK(h, hipMemsetAsync(h->part_bb, 0, nt * sizeof(double), s));
-  HIPCHK(h, hipMemsetAsync(h->part_rr, 0, 2 * nt * sizeof(double), s));
-  HIPCHK(h, hipMemsetAsync(h->part_pq, 0, nt * sizeof(double), s));
+  HIPCHK(h, hipMemsetAsync(h->part_rr, 0, nt * sizeof(double), s));
+  HIPCHK(h, hipMemsetAsync(h->part4, 0, 8 * nt * sizeof(double), s));
   HIPCHK(h, hipMemsetAsync(h->done_count, 0, sizeof(int), s));
   HIPCHK(h, hipMemsetAsync(h->iter_acc, 0, B * sizeof(unsigned long long), s));
   HIPCHK(h, hipStreamSynchronize(s));
@@ -357,13 +355,14 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
 StepArgs make_args(otfem_solver *h, double dt, int warp, const otfem_params &P) {
   StepArgs a{};
   a.x = h->x; a.x0 = h->x0; a.v = h->v; a.fext = h->fext; a.mass = h->mass; a.fixed = h->fixed;
-  a.rhs = h->rhs; a.r = h->r; a.p0 = h->p0; a.p1 = h->p1; a.y = h->y;
+  a.rhs = h->rhs; a.r = h->r; a.p = h->p; a.y = h->y;
   a.tiles = h->tiles; a.halo = h->halo; a.conn = h->conn; a.slot_tet = h->slot_tet;
   a.rest64 = h->rest64; a.vd64 = h->vd64; a.bt4 = h->bt4; a.bt8 = h->bt8;
   a.Re = h->Re; a.plastic = h->plastic; a.plastic_param = h->plastic_param;
   a.body_tile_begin = h->body_tile_begin; a.body = h->body;
-  a.part_bb = h->part_bb; a.part_rr = h->part_rr; a.part_pq = h->part_pq;
+  a.part_bb = h->part_bb; a.part_rr = h->part_rr; a.part4 = h->part4;
   a.done_count = h->done_count; a.iter_acc = h->iter_acc;
+  a.N = h->L.N;
   a.S = (int64_t)h->L.slot_tet.size();
   a.T = h->L.T;
   a.ntiles = (int32_t)h->L.tiles.size();
@@ -787,6 +786,60 @@ otfem_status otfem_set_kernel_config(otfem_handle h, int cg_block, int spmv_mode
   h->lc.spmv_mode = spmv_mode;
   h->use_graphs = use_graphs != 0;
   return OTFEM_OK;
+}
+
+otfem_status otfem_autotune(otfem_handle h, double dt, int use_stiffness_warping,
+                            const otfem_params *params, int steps, int *best_block) {
+  if (!h) return OTFEM_INVALID_ARG;
+  if (!h->initialized) return fail(h, OTFEM_STATE, "call otfem_init before otfem_autotune");
+  if (steps <= 0) steps = 2;
+  const int64_t N = h->L.N, T = h->L.T;
+  // the trial steps must leave the caller's state untouched
+  double *sx = nullptr, *sv = nullptr, *sf = nullptr, *spl = nullptr;
+  HIPCHK(h, dalloc(&sx, 3 * N));
+  HIPCHK(h, dalloc(&sv, 3 * N));
+  HIPCHK(h, dalloc(&sf, 3 * N));
+  HIPCHK(h, dalloc(&spl, 6 * T));
+  hipStream_t s = h->stream;
+  auto save = [&](bool restore) -> hipError_t {
+    hipMemcpyKind k = hipMemcpyDeviceToDevice;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(restore ? h->x : sx, restore ? sx : h->x, 3 * N * sizeof(double), k, s))) return e;
+    if ((e = hipMemcpyAsync(restore ? h->v : sv, restore ? sv : h->v, 3 * N * sizeof(double), k, s))) return e;
+    if ((e = hipMemcpyAsync(restore ? h->fext : sf, restore ? sf : h->fext, 3 * N * sizeof(double), k, s))) return e;
+    if ((e = hipMemcpyAsync(restore ? h->plastic : spl, restore ? spl : h->plastic, 6 * T * sizeof(double), k, s))) return e;
+    return hipStreamSynchronize(s);
+  };
+  otfem_status ret = OTFEM_OK;
+  const int blocks[3] = {256, 512, 1024};
+  double best = 1e300;
+  int best_b = h->lc.cg_block;
+  const bool timing_was = h->timing;
+  if (save(false) != hipSuccess) ret = fail(h, OTFEM_HIP_ERROR, "autotune: state copy failed");
+  for (int bi = 0; bi < 3 && ret == OTFEM_OK; ++bi) {
+    h->lc.cg_block = blocks[bi];
+    otfem_stats st;
+    ret = otfem_step(h, dt, use_stiffness_warping, params, &st);  // warm-up, builds the graph
+    if (ret != OTFEM_OK) break;
+    if (save(true) != hipSuccess) { ret = fail(h, OTFEM_HIP_ERROR, "autotune: restore failed"); break; }
+    h->timing = true;
+    const double e0 = h->acc_elem_ms + h->acc_cg_ms;
+    for (int k = 0; k < steps && ret == OTFEM_OK; ++k)
+      ret = otfem_step(h, dt, use_stiffness_warping, params, nullptr);
+    if (ret == OTFEM_OK) ret = sync_and_check(h, nullptr);
+    const double ms = h->acc_elem_ms + h->acc_cg_ms - e0;
+    h->timing = timing_was;
+    if (save(true) != hipSuccess) { ret = fail(h, OTFEM_HIP_ERROR, "autotune: restore failed"); break; }
+    if (ret == OTFEM_OK && ms < best) {
+      best = ms;
+      best_b = blocks[bi];
+    }
+  }
+  h->timing = timing_was;
+  h->lc.cg_block = best_b;
+  dfree(sx); dfree(sv); dfree(sf); dfree(spl);
+  if (best_block) *best_block = best_b;
+  return ret;
 }
 
 otfem_status otfem_body_stats(otfem_handle h, int *iterations, double *rel_residual) {

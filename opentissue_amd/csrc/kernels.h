@@ -33,10 +33,9 @@ struct StepArgs {
   const double *mass;
   const uint8_t *fixed;
   double *rhs;
-  double *r;
-  double *p0;
-  double *p1;
-  double *y;             // K' p of the owned nodes (consumed by the update)
+  double *r;             // [2][3N] residual, alternating by CG iteration
+  double *p;             // [2][3N] search direction
+  double *y;             // [2][3N] K' p
   // tiles / slots
   const TileInfo *tiles;
   const int32_t *halo;
@@ -53,12 +52,13 @@ struct StepArgs {
   // bodies / reductions (no atomics except done_count)
   const int32_t *body_tile_begin;
   BodyState *body;
-  double *part_bb;       // [ntiles]
-  double *part_rr;       // [2][ntiles]  r_k.r_k at [k & 1]
-  double *part_pq;       // [ntiles]
+  double *part_bb;       // [ntiles]  b.b (element kernel)
+  double *part_rr;       // [ntiles]  r_0.r_0 (element kernel)
+  double *part4;         // [2][ntiles][4]  p.q, r.q, q.q, r.r per CG kernel
   int *done_count;       // bodies whose CG stopped in this step
   unsigned long long *iter_acc;  // [B] CG iterations summed over steps
   // sizes
+  int64_t N;
   int64_t S;
   int64_t T;
   int32_t ntiles;

@@ -17,12 +17,12 @@
 //                  epilogue: right-hand side b, r = b - A v, p = r
 //                  (fem_conjugate_gradients.h:48-67), tile partials of b.b, r.r.
 //
-//   k_cg_s(it)     p = r + beta p (owned + halo), q = A p matrix-free, tile
-//                  partial of p.q (fem_conjugate_gradients.h:72-99, 124-131).
-//   k_cg_u(it)     alpha = r.r / p.q, v += alpha p, r -= alpha q, tile partial
-//                  of r.r (:101-117).
-//   k_finish       records the stop of bodies that reached the cap, then
-//                  x += dt v (fem_position_update.h:25-39).
+//   k_cg_iter(it)  one CG iteration per kernel: completes iteration it-1
+//                  (alpha, v += alpha p, r -= alpha q), p = r + beta p for owned
+//                  and halo nodes, q = A p matrix-free, tile partials of p.q,
+//                  r.q, q.q, r.r (fem_conjugate_gradients.h:69-132).
+//   k_finish       completes the last iteration of bodies that reached the
+//                  cap, then x += dt v (fem_position_update.h:25-39).
 //
 // No workgroup waits for another inside a kernel: the kernel boundary is the
 // grid barrier (cheaper on this GPU than any in-kernel barrier over hundreds
@@ -91,6 +91,48 @@ __device__ __forceinline__ double body_sum(const double *part, int t0, int t1, d
   double s = 0.0;
   for (int t = t0 + (int)threadIdx.x; t < t1; t += BLOCK) s += part[t];
   return block_sum<BLOCK>(s, red);
+}
+
+// Four sums at once (one tree, one set of barriers); result broadcast.
+// `red4` holds 4 * (BLOCK/64 + 1) doubles of LDS.
+template <int BLOCK>
+__device__ __forceinline__ double4 block_sum4(double4 a, double *red4) {
+  constexpr int W = BLOCK / 64;
+  a.x = wave_sum(a.x);
+  a.y = wave_sum(a.y);
+  a.z = wave_sum(a.z);
+  a.w = wave_sum(a.w);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) {
+    red4[4 * w] = a.x;
+    red4[4 * w + 1] = a.y;
+    red4[4 * w + 2] = a.z;
+    red4[4 * w + 3] = a.w;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double s = 0.0;
+    for (int i = 0; i < W; ++i) s += red4[4 * i + threadIdx.x];
+    red4[4 * W + threadIdx.x] = s;
+  }
+  __syncthreads();
+  double4 r = make_double4(red4[4 * W], red4[4 * W + 1], red4[4 * W + 2], red4[4 * W + 3]);
+  __syncthreads();
+  return r;
+}
+
+// All four fields of one body's [tile][4] partial records, in tile order.
+template <int BLOCK>
+__device__ __forceinline__ double4 body_sum4(const double4 *part4, int t0, int t1, double *red4) {
+  double4 s = make_double4(0.0, 0.0, 0.0, 0.0);
+  for (int t = t0 + (int)threadIdx.x; t < t1; t += BLOCK) {
+    const double4 v = part4[t];
+    s.x += v.x;
+    s.y += v.y;
+    s.z += v.z;
+    s.w += v.w;
+  }
+  return block_sum4<BLOCK>(s, red4);
 }
 
 __device__ __forceinline__ bool finite(double x) { return isfinite(x); }
@@ -411,7 +453,6 @@ __global__ __launch_bounds__(kElemBlock) void k_element(StepArgs A) {
       for (int d = 0; d < 3; ++d) {
         A.rhs[3 * g + d] = 0.0;
         A.r[3 * g + d] = 0.0;
-        A.p0[3 * g + d] = 0.0;
       }
       continue;
     }
@@ -422,7 +463,6 @@ __global__ __launch_bounds__(kElemBlock) void k_element(StepArgs A) {
       double rd = bd - (A.cM * m * vel + A.cK * skv[3 * l + d]);
       A.rhs[3 * g + d] = bd;
       A.r[3 * g + d] = rd;
-      A.p0[3 * g + d] = rd;
       bb += bd * bd;
       rr += rd * rd;
     }
@@ -549,73 +589,112 @@ __device__ __forceinline__ void slot_spmv(ushort4 c, const MT bt[3][3], const MT
 
 }  // namespace
 
-// p = r + beta p; q = A p; partial p.q.   MT: element arithmetic type, AT:
-// LDS accumulation type of K'p.
+// One CG iteration per kernel (fem_conjugate_gradients.h:69-132).  Kernel j
+// first completes iteration j-1 -- alpha = r.r / p.Ap from the previous
+// kernel's partials (:101-105), v += alpha p (:113), r -= alpha q (:114) --
+// then forms p = r + beta p (:124-131) for owned and halo nodes, applies
+// q = A p matrix-free, and leaves partial sums of p.q, r.q, q.q and r.r.
+// r.r after the update comes from the exact recurrence
+// r'.r' = r.r - 2 alpha r.q + alpha^2 q.q, so the stop test and beta need no
+// extra pass; alpha itself always uses the directly summed r.r.
+// Buffers alternate with j: kernel j reads set (j-1)&1 and writes set j&1.
 template <int BLOCK, typename MT, typename AT, typename ST>
-__global__ __launch_bounds__(BLOCK) void k_cg_s(StepArgs A, int it) {
+__global__ __launch_bounds__(BLOCK) void k_cg_iter(StepArgs A, int it) {
   extern __shared__ double lds[];
-  __shared__ double red[BLOCK / 64 + 1];
+  __shared__ double red[4 * (BLOCK / 64 + 1)];
   if (*(volatile int *)A.done_count >= A.B) return;
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const TileInfo ti = A.tiles[tile];
   BodyState *bs = &A.body[ti.body];
   if (*(volatile int32_t *)&bs->stopped) return;
   const int t0 = A.body_tile_begin[ti.body], t1 = A.body_tile_begin[ti.body + 1];
-  const double rr = body_sum<BLOCK>(A.part_rr + (it & 1) * A.ntiles, t0, t1, red);
-  double bb;
-  if (it == 0)
+  const int nt = A.ntiles;
+  const double *prevp = A.part4 + ((it - 1) & 1) * 4 * (int64_t)nt;
+
+  double alpha = 0.0, beta = 0.0, rr, bb;
+  if (it == 0) {
+    rr = body_sum<BLOCK>(A.part_rr, t0, t1, red);
     bb = body_sum<BLOCK>(A.part_bb, t0, t1, red);
-  else
+  } else {
+    const double4 s4 = body_sum4<BLOCK>(reinterpret_cast<const double4 *>(prevp), t0, t1, red);
+    double pq = s4.x;
+    const double rq = s4.y, qq = s4.z, rrd = s4.w;
     bb = bs->bb;
-  bool conv, nonfin;
-  if (stop_rule(A, it, rr, bb, conv, nonfin)) {
-    if (ti.body_rank == 0 && threadIdx.x == 0) {
-      if (it == 0) {
-        bs->bb = bb;
-        bs->rr0 = rr;
-      }
-      record_stop(A, bs, it, rr, conv, nonfin);
+    if (fabs(pq) < A.tiny) pq = A.tiny;  // fem_conjugate_gradients.h:101-102
+    alpha = rrd / pq;
+    if (!finite(alpha)) {
+      if (ti.body_rank == 0 && threadIdx.x == 0) record_stop(A, bs, it - 1, rrd, false, true);
+      return;
     }
-    return;
+    rr = fmax(rrd - 2.0 * alpha * rq + alpha * alpha * qq, 0.0);
+    double d = rrd;
+    if (fabs(d) < A.tiny) d = A.tiny;  // fem_conjugate_gradients.h:121-122
+    beta = rr / d;
   }
-  double beta = 0.0;
-  if (it > 0) {
-    double rprev = body_sum<BLOCK>(A.part_rr + ((it - 1) & 1) * A.ntiles, t0, t1, red);
-    if (fabs(rprev) < A.tiny) rprev = A.tiny;  // fem_conjugate_gradients.h:121-122
-    beta = rr / rprev;
-  }
+  bool conv, nonfin;
+  const bool stop = stop_rule(A, it, rr, bb, conv, nonfin);
   if (ti.body_rank == 0 && threadIdx.x == 0) {
     if (it == 0) {
       bs->bb = bb;
       bs->rr0 = rr;
     }
-    bs->d = rr;
-    bs->iterations = it;
+    if (stop)
+      record_stop(A, bs, it, rr, conv, nonfin);
+    else
+      bs->iterations = it;
   }
 
-  const double *pin = (it & 1) ? A.p1 : A.p0;
-  double *pout = (it & 1) ? A.p0 : A.p1;
   const int nn = ti.n_owned + ti.n_halo, no = ti.n_owned;
-  double *sp = lds;
-  AT *sy = reinterpret_cast<AT *>(sp + 3 * nn);
+  const int cur = it & 1, prv = (it - 1) & 1;
+  double *Rn = A.r + cur * 3 * A.N, *Pn = A.p + cur * 3 * A.N, *Yn = A.y + cur * 3 * A.N;
+  const double *Rp = A.r + prv * 3 * A.N, *Pp = A.p + prv * 3 * A.N, *Yp = A.y + prv * 3 * A.N;
+  double *sp = lds;                 // p of every tile node
+  double *sr = sp + 3 * nn;         // r of owned nodes
+  AT *sy = reinterpret_cast<AT *>(sr + 3 * no);  // K'p of owned nodes
 
+  // prologue: finish iteration it-1 and form the new direction
   for (int l = threadIdx.x; l < nn; l += BLOCK) {
-    const int64_t g = l < no ? ti.node_begin + l : A.halo[ti.halo_begin + l - no];
+    const int64_t g = l < no ? ti.node_begin + l : __ldg(&A.halo[ti.halo_begin + l - no]);
+    const bool fx = A.fixed[g] != 0;
+    double rn[3], pn[3];
+    if (it == 0) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      double pn = A.r[3 * g + d] + beta * pin[3 * g + d];
-      sp[3 * l + d] = pn;
-      if (l < no) {
-        pout[3 * g + d] = pn;
+      for (int d = 0; d < 3; ++d) {
+        rn[d] = fx ? 0.0 : A.r[3 * g + d];
+        pn[d] = rn[d];
+      }
+    } else if (fx) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) rn[d] = pn[d] = 0.0;
+    } else {
+      const double m = A.mass[g];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double pk = Pp[3 * g + d];
+        const double q = A.cM * m * pk + A.cK * Yp[3 * g + d];
+        rn[d] = Rp[3 * g + d] - alpha * q;
+        pn[d] = rn[d] + beta * pk;
+        if (l < no) A.v[3 * g + d] += alpha * pk;
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) sp[3 * l + d] = pn[d];
+    if (l < no) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (it > 0 && !stop) Rn[3 * g + d] = rn[d];
+        Pn[3 * g + d] = pn[d];
+        sr[3 * l + d] = rn[d];
         sy[3 * l + d] = (AT)0;
       }
     }
   }
+  if (stop) return;
   __syncthreads();
 
+  // q = K' p over the tile's slots, two slots per thread in flight
   const int64_t S = A.S;
   const int send = ti.slot_begin + ti.n_slots;
-  // two slots per thread per trip: both slots' loads are in flight together
   for (int s = ti.slot_begin + threadIdx.x; s < send; s += 2 * BLOCK) {
     const int s2 = s + BLOCK;
     const bool two = s2 < send;
@@ -635,88 +714,74 @@ __global__ __launch_bounds__(BLOCK) void k_cg_s(StepArgs A, int it) {
   }
   __syncthreads();
 
-  double pq = 0.0;
+  double pq = 0.0, rq = 0.0, qq = 0.0, rrd = 0.0;
   for (int l = threadIdx.x; l < no; l += BLOCK) {
     const int64_t g = ti.node_begin + l;
     if (A.fixed[g]) continue;
     const double m = A.mass[g];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      double yd = (double)sy[3 * l + d];
-      A.y[3 * g + d] = yd;
-      double p = sp[3 * l + d];
-      double q = A.cM * m * p + A.cK * yd;
+      const double yd = (double)sy[3 * l + d];
+      Yn[3 * g + d] = yd;
+      const double p = sp[3 * l + d], r = sr[3 * l + d];
+      const double q = A.cM * m * p + A.cK * yd;
       pq += p * q;
+      rq += r * q;
+      qq += q * q;
+      rrd += r * r;
     }
   }
-  pq = block_sum<BLOCK>(pq, red);
-  if (threadIdx.x == 0) A.part_pq[tile] = pq;
+  const double4 tot = block_sum4<BLOCK>(make_double4(pq, rq, qq, rrd), red);
+  if (threadIdx.x == 0) reinterpret_cast<double4 *>(A.part4 + cur * 4 * (int64_t)nt)[tile] = tot;
 }
 
-// alpha = d / p.q (fem_conjugate_gradients.h:101-105); v += alpha p;
-// r -= alpha q (:110-117); partial r.r.
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_cg_u(StepArgs A, int it) {
-  __shared__ double red[BLOCK / 64 + 1];
-  if (*(volatile int *)A.done_count >= A.B) return;
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const TileInfo ti = A.tiles[tile];
-  BodyState *bs = &A.body[ti.body];
-  if (*(volatile int32_t *)&bs->stopped) return;
-  const int t0 = A.body_tile_begin[ti.body], t1 = A.body_tile_begin[ti.body + 1];
-  double d2 = body_sum<BLOCK>(A.part_pq, t0, t1, red);
-  if (fabs(d2) < A.tiny) d2 = A.tiny;  // fem_conjugate_gradients.h:101-102
-  const double d = bs->d;
-  const double alpha = d / d2;
-  if (!finite(alpha)) {
-    if (ti.body_rank == 0 && threadIdx.x == 0) record_stop(A, bs, it, d, false, true);
-    return;
-  }
-  const double *pn = (it & 1) ? A.p0 : A.p1;
-  double rr = 0.0;
-  for (int l = threadIdx.x; l < ti.n_owned; l += BLOCK) {
-    const int64_t g = ti.node_begin + l;
-    if (A.fixed[g]) continue;
-    const double m = A.mass[g];
-#pragma unroll
-    for (int d3 = 0; d3 < 3; ++d3) {
-      double p = pn[3 * g + d3];
-      double q = A.cM * m * p + A.cK * A.y[3 * g + d3];
-      A.v[3 * g + d3] += alpha * p;
-      double rd = A.r[3 * g + d3] - alpha * q;
-      A.r[3 * g + d3] = rd;
-      rr += rd * rd;
-    }
-  }
-  rr = block_sum<BLOCK>(rr, red);
-  if (threadIdx.x == 0) A.part_rr[((it + 1) & 1) * A.ntiles + tile] = rr;
-}
-
-// Bodies still running after cg_max iterations stop here; then x += dt v.
+// Bodies still running after cg_max iterations complete their last update
+// here (v += alpha p) and record their stop; then x += dt v
+// (fem_position_update.h:33-38).
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_finish(StepArgs A) {
-  __shared__ double red[BLOCK / 64 + 1];
+  __shared__ double red[4 * (BLOCK / 64 + 1)];
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const TileInfo ti = A.tiles[tile];
   BodyState *bs = &A.body[ti.body];
-  if (ti.body_rank == 0) {
-    const int stopped = *(volatile int32_t *)&bs->stopped;
-    if (!stopped) {
-      const int t0 = A.body_tile_begin[ti.body], t1 = A.body_tile_begin[ti.body + 1];
-      const int it = A.cg_max;
-      const double rr = body_sum<BLOCK>(A.part_rr + (it & 1) * A.ntiles, t0, t1, red);
-      if (threadIdx.x == 0) {
-        bool conv, nonfin;
-        stop_rule(A, it, rr, bs->bb, conv, nonfin);
-        bs->iterations = it;
-        bs->rr = rr;
-        bs->converged = conv ? 1 : 0;
-        bs->flags |= nonfin ? 1 : 0;
-        bs->stopped = 1;
+  const int stopped = *(volatile int32_t *)&bs->stopped;
+  const int it = A.cg_max;
+  if (!stopped && it > 0) {
+    const int t0 = A.body_tile_begin[ti.body], t1 = A.body_tile_begin[ti.body + 1];
+    const double *prevp = A.part4 + ((it - 1) & 1) * 4 * (int64_t)A.ntiles;
+    const double4 s4 = body_sum4<BLOCK>(reinterpret_cast<const double4 *>(prevp), t0, t1, red);
+    double pq = s4.x;
+    const double rq = s4.y, qq = s4.z, rrd = s4.w;
+    if (fabs(pq) < A.tiny) pq = A.tiny;
+    const double alpha = rrd / pq;
+    const bool ok = finite(alpha);
+    if (ok) {
+      const double *Pp = A.p + ((it - 1) & 1) * 3 * A.N;
+      for (int l = threadIdx.x; l < ti.n_owned; l += BLOCK) {
+        const int64_t g = ti.node_begin + l;
+        if (A.fixed[g]) continue;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) A.v[3 * g + d] += alpha * Pp[3 * g + d];
       }
     }
-    if (threadIdx.x == 0) A.iter_acc[ti.body] += (unsigned long long)bs->iterations;
+    if (ti.body_rank == 0 && threadIdx.x == 0) {
+      const double rr = ok ? fmax(rrd - 2.0 * alpha * rq + alpha * alpha * qq, 0.0) : rrd;
+      bool conv, nonfin;
+      stop_rule(A, it, rr, bs->bb, conv, nonfin);
+      bs->iterations = it;
+      bs->rr = rr;
+      bs->converged = conv ? 1 : 0;
+      bs->flags |= (nonfin || !ok) ? 1 : 0;
+      bs->stopped = 1;
+    }
+    __syncthreads();
+  } else if (!stopped && ti.body_rank == 0 && threadIdx.x == 0) {
+    bs->iterations = 0;
+    bs->rr = bs->rr0;
+    bs->stopped = 1;
   }
+  if (ti.body_rank == 0 && threadIdx.x == 0)
+    A.iter_acc[ti.body] += (unsigned long long)(*(volatile int32_t *)&bs->iterations);
   for (int l = threadIdx.x; l < ti.n_owned; l += BLOCK) {
     const int64_t g = ti.node_begin + l;
     if (A.fixed[g]) continue;
@@ -730,18 +795,18 @@ size_t element_lds_bytes(int max_nodes, int max_owned, int plastic_on) {
 }
 
 size_t cg_lds_bytes(int max_nodes, int max_owned) {
-  // FP64 p for all tile nodes + K'p accumulators (FP64 covers every mode)
-  return sizeof(double) * 3 * (size_t)max_nodes + sizeof(double) * 3 * (size_t)max_owned;
+  // FP64 p of all tile nodes, FP64 r of owned nodes, K'p accumulators
+  // (FP64 covers every mode)
+  return sizeof(double) * 3 * (size_t)max_nodes + 2 * sizeof(double) * 3 * (size_t)max_owned;
 }
 
 namespace {
 
 template <int BLOCK, typename MT, typename AT, typename ST>
 hipError_t enqueue_iterations(const StepArgs &a, size_t lds_cg, hipStream_t s) {
-  for (int it = 0; it < a.cg_max; ++it) {
-    hipLaunchKernelGGL((k_cg_s<BLOCK, MT, AT, ST>), dim3(a.ntiles), dim3(BLOCK), lds_cg, s, a, it);
-    hipLaunchKernelGGL((k_cg_u<BLOCK>), dim3(a.ntiles), dim3(BLOCK), 0, s, a, it);
-  }
+  for (int it = 0; it < a.cg_max; ++it)
+    hipLaunchKernelGGL((k_cg_iter<BLOCK, MT, AT, ST>), dim3(a.ntiles), dim3(BLOCK), lds_cg, s, a,
+                       it);
   hipLaunchKernelGGL((k_finish<BLOCK>), dim3(a.ntiles), dim3(BLOCK), 0, s, a);
   return hipGetLastError();
 }

@@ -50,7 +50,11 @@ std::string build_layout(int64_t N, int64_t T, const double *coords, const int32
   L = Layout();
   if (N <= 0 || T < 0) return "num_nodes must be positive and num_tets non-negative";
   if (N >= (int64_t)INT32_MAX || T >= (int64_t)INT32_MAX) return "mesh too large for 32-bit ids";
-  if (target_tile_nodes <= 0) target_tile_nodes = 512;
+  // Default tiling: about 512 owned nodes per tile; a body large enough to
+  // fill the GPU gets a tile count that is a multiple of the 256 compute
+  // units of an MI355X, so every CU runs the same number of tiles.
+  const bool auto_tiles = target_tile_nodes <= 0;
+  if (auto_tiles) target_tile_nodes = 512;
   if (max_tile_nodes <= 0) max_tile_nodes = 8192;
   L.N = N;
   L.T = T;
@@ -97,6 +101,11 @@ std::string build_layout(int64_t N, int64_t T, const double *coords, const int32
     std::vector<int32_t> idx(n1 - n0);
     std::iota(idx.begin(), idx.end(), (int32_t)n0);
     int64_t parts = std::max<int64_t>(1, (n1 - n0 + target_tile_nodes - 1) / target_tile_nodes);
+    if (auto_tiles && L.B == 1) {
+      const int64_t cus = 256;
+      const int64_t k = (int64_t)std::llround((double)(n1 - n0) / (512.0 * cus));
+      if (k >= 1) parts = k * cus;
+    }
     std::vector<int32_t> sizes;
     rcb(idx, 0, (int64_t)idx.size(), parts, coords, sizes);
     int64_t pos = 0;

@@ -109,6 +109,8 @@ SIGNATURES = {
     "otfem_get_timing": (C.c_int, [_H, _dp, _dp, C.POINTER(C.c_longlong)]),
     "otfem_iteration_totals": (C.c_int, [_H, C.POINTER(C.c_int64), C.c_int]),
     "otfem_set_kernel_config": (C.c_int, [_H, C.c_int, C.c_int, C.c_int]),
+    "otfem_autotune": (C.c_int, [_H, C.c_double, C.c_int, C.POINTER(Params), C.c_int,
+                                 C.POINTER(C.c_int)]),
     "otfem_body_stats": (C.c_int, [_H, C.POINTER(C.c_int), _dp]),
     "otfem_update_original_coord": (C.c_int, [_H]),
     "otfem_export_K": (C.c_int, [_H, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
@@ -302,6 +304,14 @@ class Solver:
 
     def set_kernel_config(self, cg_block=256, spmv_mode=1, use_graphs=True):
         self._check(self.lib.otfem_set_kernel_config(self.h, cg_block, spmv_mode, 1 if use_graphs else 0))
+
+    def autotune(self, dt, warp=True, params=None, steps=2):
+        """Pick the fastest CG block size for this mesh (state is restored)."""
+        P = reference_params() if params is None else params
+        best = C.c_int()
+        self._check(self.lib.otfem_autotune(self.h, dt, 1 if warp else 0, C.byref(P), steps,
+                                            C.byref(best)))
+        return best.value
 
     def body_stats(self):
         it = np.empty(self.B, dtype=np.int32)

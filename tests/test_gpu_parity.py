@@ -100,3 +100,178 @@ def test_rotations_match_oracle():
     Rg, Ro = s.rotations(), o.rotations()
     assert np.abs(Rg - Ro).max() < 1e-12
     assert np.abs(np.einsum("tij,tkj->tik", Rg, Rg) - np.eye(3)).max() < 1e-13
+
+
+def _bodies(dims_list, seed=0, jitter=0.2):
+    from opentissue_amd import mesh
+    rng = np.random.default_rng(seed)
+    parts, mats, forces, fixed = [], [], [], []
+    for b, d in enumerate(dims_list):
+        c, t = mesh.generate_blocks(*d, 0.01, 0.01, 0.01)
+        if jitter:
+            c = mesh.jitter_interior(c, t, 0.01, jitter, seed * 100 + b)
+        fx = mesh.face_nodes(c, 0, 0).astype(np.uint8)
+        E, nu = float(np.exp(rng.uniform(np.log(1e4), np.log(1e6)))), float(rng.uniform(0.25, 0.45))
+        f = mesh.traction_forces(c, t, 0, 1, rng.normal(size=3) * 1e3)
+        f[fx == 1] = 0
+        parts.append((c, t))
+        mats.append((E, nu))
+        forces.append(f)
+        fixed.append(fx)
+    return parts, mats, forces, fixed
+
+
+@pytest.mark.parametrize("mode", ["ref", "cfg"])
+def test_batched_bodies_match_per_body_oracle(mode):
+    """Bodies of uneven size stepped together; each has its own CG
+    (convergence mask), so each must match an oracle run of that body alone."""
+    from opentissue_amd import fem, mesh
+    import oracle
+    dims = [(2, 3, 2), (3, 3, 3), (1, 2, 4), (5, 2, 2), (1, 1, 1)]
+    parts, mats, forces, fixed = _bodies(dims)
+    coords, tets, no, to = mesh.pack_bodies(parts)
+    E = np.concatenate([np.full(len(t), m[0]) for (c, t), m in zip(parts, mats)])
+    nu = np.concatenate([np.full(len(t), m[1]) for (c, t), m in zip(parts, mats)])
+    s = fem.Solver(coords, tets, np.concatenate(fixed), body_node_offsets=no, body_tet_offsets=to,
+                   target_tile_nodes=16)
+    s.init_materials(E, nu, 1000.0)
+    s.set_external_forces(np.concatenate(forces))
+    gp = fem.reference_params() if mode == "ref" else fem.config_params()
+    op = oracle.reference_params() if mode == "ref" else oracle.config_params()
+    orcs = []
+    for (c, t), (Eb, nub), f, fx in zip(parts, mats, forces, fixed):
+        o = oracle.Oracle(c, t, fx)
+        o.init(Eb, nub, 1000.0)
+        o.set_external_forces(f)
+        orcs.append(o)
+    for step in range(4):
+        xo = np.concatenate([o.state()[0] for o in orcs])
+        vo = np.concatenate([o.state()[1] for o in orcs])
+        s.set_state(xo, vo)
+        s.simulate(0.01, True, gp)
+        its_g, _ = s.body_stats()
+        xg, _ = s.state()
+        for b, o in enumerate(orcs):
+            st = o.simulate(0.01, True, op)
+            x1, _ = o.state()
+            sl = slice(no[b], no[b + 1])
+            err = rel_disp_error(xg[sl], x1, parts[b][0])
+            assert err <= TOL, (step, b, err)
+            assert abs(int(its_g[b]) - st.iterations) <= 2, (step, b, its_g[b], st.iterations)
+
+
+def test_polar_rotation_mode():
+    from opentissue_amd import fem
+    import oracle
+    c, t, fx = _lattice(3, 6, 2)
+    s, o = _pair(c, t, fx)
+    gp = fem.config_params(rotation_mode=1, store_rotations=1)
+    op = oracle.config_params(rotation_mode=1)
+    errs, _ = _per_step_errors(s, o, c, 4, 0.01, gp, op)
+    assert errs.max() <= TOL, errs
+    R = s.rotations()
+    assert np.abs(R - o.rotations()).max() < 1e-9
+    assert np.all(np.linalg.det(R) > 0)
+
+
+def test_no_stiffness_warping():
+    from opentissue_amd import fem
+    import oracle
+    c, t, fx = _lattice(3, 6, 2)
+    s, o = _pair(c, t, fx)
+    errs, _ = _per_step_errors(s, o, c, 4, 0.01, fem.config_params(), oracle.config_params(),
+                               warp=False)
+    assert errs.max() <= TOL, errs
+
+
+def test_plasticity_matches_oracle():
+    from opentissue_amd import fem
+    import oracle
+    c, t, fx = _lattice(2, 5, 2)
+    s = fem.Solver(c, t, fx)
+    o = oracle.Oracle(c, t, fx)
+    s.init(5e5, 0.33, 1000.0, 1e-4, 20.0, 0.05)
+    o.init(5e5, 0.33, 1000.0, 1e-4, 20.0, 0.05)
+    m = o.mass()
+    from opentissue_amd import mesh
+    f = mesh.gravity_forces(np.where(fx == 1, 0.0, m), fx) * 50.0
+    errs = []
+    for _ in range(5):
+        # the reference accumulates plastic forces into f_external: reset it
+        s.set_external_forces(f)
+        o.set_external_forces(f)
+        xo, vo = o.state()
+        s.set_state(xo, vo)
+        s.simulate(0.01, True, fem.config_params())
+        o.simulate(0.01, True, oracle.config_params())
+        errs.append(rel_disp_error(s.state()[0], o.state()[0], c))
+        # both must have accumulated the same plastic forces into f_external
+        np.testing.assert_allclose(s.external_forces(), o.external_forces(), rtol=1e-6,
+                                   atol=1e-9 * np.abs(f).max())
+    assert max(errs) <= TOL, errs
+    assert np.abs(o.plastic()).max() > 0
+    np.testing.assert_allclose(s.plastic(), o.plastic(), rtol=1e-5, atol=1e-9)
+
+
+def test_fixed_node_velocity_enters_warm_start():
+    # r = b - A v uses every column, fixed ones included (fem_conjugate_gradients.h:52-66)
+    from opentissue_amd import fem
+    import oracle
+    c, t, fx = _lattice(2, 4, 2)
+    rng = np.random.default_rng(3)
+    v0 = 0.05 * rng.normal(size=c.shape)
+    s = fem.Solver(c, t, fx, velocity=v0)
+    o = oracle.Oracle(c, t, fx, velocity=v0)
+    s.init(5e5, 0.33, 1000.0)
+    o.init(5e5, 0.33, 1000.0)
+    s.simulate(0.01, True, fem.config_params())
+    o.simulate(0.01, True, oracle.config_params())
+    xg, vg = s.state()
+    xo, vo = o.state()
+    assert rel_disp_error(xg, xo, c) <= TOL
+    # fixed nodes neither move nor change velocity
+    np.testing.assert_array_equal(vg[fx == 1], v0[fx == 1])
+    np.testing.assert_array_equal(xg[fx == 1], c[fx == 1])
+
+
+def test_export_K_matches_oracle():
+    from opentissue_amd import fem
+    import oracle
+    c, t, fx = _lattice(2, 3, 2)
+    s, o = _pair(c, t, fx)
+    P = fem.reference_params()
+    P.store_rotations = 1
+    for _ in range(3):
+        xo, vo = o.state()
+        s.set_state(xo, vo)
+        s.simulate(0.01, True, P)
+        o.simulate(0.01, True, oracle.reference_params())
+    rp, cols, blocks = s.export_K()
+    N = len(c)
+    K = np.zeros((3 * N, 3 * N))
+    for i in range(N):
+        for k in range(rp[i], rp[i + 1]):
+            K[3 * i:3 * i + 3, 3 * cols[k]:3 * cols[k] + 3] = blocks[k]
+    Ko = o.export_K_dense()
+    np.testing.assert_allclose(K, Ko, rtol=0, atol=1e-9 * np.abs(Ko).max())
+
+
+def test_update_original_coord_then_reinit():
+    from opentissue_amd import fem
+    import oracle
+    c, t, fx = _lattice(2, 3, 2)
+    s, o = _pair(c, t, fx)
+    for _ in range(3):
+        s.simulate(0.01, True, fem.config_params())
+        o.simulate(0.01, True, oracle.config_params())
+    s.update_original_coord()
+    o.update_original_coord()
+    with pytest.raises(fem.OtfemError):
+        s.simulate(0.01, True, fem.config_params())  # stale until init (fem_update_original_coord.h:19-23)
+    s.init(5e5, 0.33, 1000.0)
+    o.init(5e5, 0.33, 1000.0)
+    x1, v1 = o.state()
+    s.set_state(x1, v1)
+    s.simulate(0.01, True, fem.config_params())
+    o.simulate(0.01, True, oracle.config_params())
+    assert rel_disp_error(s.state()[0], o.state()[0], x1) <= TOL
