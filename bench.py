@@ -9,10 +9,15 @@ A step is one full implicit step (rotations + elastic forces + right-hand
 side + CG + position update) over the whole mesh, inputs resident in HBM.
 
 Output: one JSON line.  `value` is tet-steps/s over exactly --steps timed
-steps from rest; `roofline` prices the dominant kernels (one k_cg_iter launch
-per CG iteration, plus k_finish) by their algorithmic bytes (100 B/tet + 195 B/node per CG iteration,
-FP64 v and r, plus 36 B/node for the position update) against the MI355X HBM
-peak; `cpu_baseline` times the FP64 CPU restatement of the reference step
+steps from rest with the plain CG start (the reference's v^n); `learned`
+repeats the timed steps with the learned CG start (network trained first, on
+2,000 plain-solver steps of the 10x10x40 cantilever under seeded random
+tractions; training is not timed) and gives the learned-vs-plain A/B on
+held-out cantilever episodes with CG run to rel-res 1e-6 uncapped.
+`roofline` prices the dominant kernels (one k_cg_iter launch per CG
+iteration, plus k_finish) by their algorithmic bytes (100 B/tet + 195 B/node
+per CG iteration, FP64 v and r, plus 36 B/node for the position update)
+against the MI355X HBM peak; `cpu_baseline` times the FP64 CPU restatement of the reference step
 (one thread) on a bounded sample of the same workload.
 
     python bench.py [--steps 200] [--warmup 5] [--config cube64|cantilever|bodies]
@@ -96,6 +101,46 @@ def run_cpu_baseline(w, params_kw, seconds_budget):
     return dict(seconds=t, init_seconds=t_init, iterations=st.iterations, x=x, v=v)
 
 
+def train_warmstart_model():
+    """Network for the learned CG start: 2,000 plain-solver steps (20 episodes
+    of 100) of the 10x10x40 cantilever under seeded random tractions, then
+    Adam on the residual loss.  Returns (model, solver, workload, info)."""
+    from opentissue_amd import fem, warmstart, workloads
+    w = workloads.cantilever()
+    s = warmstart.make_solver(w)
+    P = fem.config_params()
+    model = fem.Model(seed=0)
+    t0 = time.perf_counter()
+    loads, _ = warmstart.random_loads(w, 20, seed=1)
+    n = warmstart.record(s, model, w, loads, 100, P)
+    t1 = time.perf_counter()
+    loss = model.train_residual(s, w.dt, P, steps=250, lr=1e-4, seed=0)
+    t2 = time.perf_counter()
+    info = dict(samples=int(n), record_seconds=t1 - t0, train_seconds=t2 - t1, adam_steps=250,
+                lr=1e-4, nodes_per_batch=int(s.n_free * max(1, round(65536 / s.n_free))),
+                final_loss=float(np.mean(loss[-50:])))
+    return model, s, w, info
+
+
+def learned_ab(model, s, w):
+    """Held-out cantilever episodes (seed 1001, 3 x 100 steps from rest): plain
+    vs learned start, CG to rel-res 1e-6 with cap 3000 (every step converges)."""
+    from opentissue_amd import fem, warmstart
+    U = fem.config_params(cap=3000)
+    loads, _ = warmstart.random_loads(w, 3, seed=1001)
+    plain = warmstart.run_episodes(s, w, loads, 100, U, learned=False)
+    learned = warmstart.run_episodes(s, w, loads, 100, U, learned=True, model=model)
+    return {
+        "workload": f"{w.name} ({w.num_tets} tets), 3 held-out episodes x 100 steps, "
+                    "random traction 0.5e3..2e3 Pa, CG rel-res 1e-6 cap 3000",
+        "plain": {"tet_steps_per_s": plain["tet_steps_per_s"],
+                  "cg_iterations_per_step": plain["mean_iterations"]},
+        "learned": {"tet_steps_per_s": learned["tet_steps_per_s"],
+                    "cg_iterations_per_step": learned["mean_iterations"]},
+        "speedup": learned["tet_steps_per_s"] / plain["tet_steps_per_s"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +151,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=60.0)
     ap.add_argument("--tile-nodes", type=int, default=0)
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm start legs")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -171,7 +217,7 @@ def main():
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
     result = {
-        "metric": "tet-steps/s at CG rel-res 1e-6 (plain warm start); CG HBM GB/s % of peak",
+        "metric": "tet-steps/s at CG rel-res 1e-6, learned vs plain; CG HBM GB/s % of peak",
         "value": value,
         "unit": "tet-steps/s",
         "n_gpus": 1,
@@ -204,6 +250,44 @@ def main():
         },
         "cpu_baseline": None,
     }
+
+    if not args.no_learned:
+        from opentissue_amd import fem as F
+        log("learned warm start: recording 2,000 cantilever steps and training (not timed)")
+        model, cs, cw, info = train_warmstart_model()
+        ab = learned_ab(model, cs, cw)
+        cs.close()
+        # the timed steps again, CG started from the network
+        PL = F.Params.from_buffer_copy(P)
+        PL.warm_start = F.WARM_LEARNED
+        s.set_warmstart_model(model)
+        s.set_state(w.coords, zeros)
+        s.iteration_totals(reset=True)
+        s.set_timing(True)
+        s.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            s.step_async(w.dt, True, PL)
+        stl = s.synchronize()
+        wl = time.perf_counter() - t0
+        itl = s.iteration_totals()
+        el_ms, cgl_ms, nl = s.timing()
+        s.set_warmstart_model(None)
+        vl = T * args.steps / wl
+        log(f"learned start: {vl:.4g} tet-steps/s ({vl / value:.3f}x plain), "
+            f"{itl.sum() / args.steps / w.num_bodies:.1f} CG iterations per step; "
+            f"A/B on held-out cantilever episodes (uncapped): {ab['speedup']:.3f}x")
+        result["learned"] = {
+            "value": vl,
+            "unit": "tet-steps/s",
+            "vs_plain": vl / value,
+            "cg_iterations_per_step": float(itl.sum()) / args.steps / w.num_bodies,
+            "kernel_ms_per_step": {"element_and_network": el_ms / max(1, nl),
+                                   "cg": cgl_ms / max(1, nl)},
+            "flags": int(stl.flags),
+            "training": info,
+            "ab_uncapped": ab,
+        }
 
     if not args.no_cpu:
         # bounded sample: one step from rest of the same workload; the GPU's

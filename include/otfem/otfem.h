@@ -71,7 +71,8 @@ typedef struct {
   int iterations;        /* max CG iterations over bodies this step */
   int bodies_converged;  /* bodies whose stopping rule fired */
   int bodies_capped;     /* bodies that reached cg_max_iterations unconverged */
-  int flags;             /* OR of per-body status bits (1: non-finite) */
+  int flags;             /* 1: a CG scalar became non-finite; 2: the learned start
+                            was not finite at some node (that node used v^n) */
   double rr0;            /* body 0: r.r after the warm start */
   double rr;             /* body 0: final recursive r.r */
   double bb;             /* body 0: b.b over free dofs */
@@ -180,6 +181,91 @@ otfem_status otfem_export_K(otfem_handle h, int64_t *nnz_blocks, int64_t *row_pt
 /* Layout facts (no device work): tiles, slots (tet copies) and halo nodes. */
 otfem_status otfem_layout_info(otfem_handle h, int64_t *num_tiles, int64_t *num_slots,
                                int64_t *num_halo, int64_t *max_tile_nodes);
+
+/* ------------------------------------------------------------------------
+ * Learned CG warm start.  No reference counterpart: the reference's CG starts
+ * from m_velocity (fem_conjugate_gradients.h:48-67).  A per-node network
+ * 12 -> 128 -> 128 -> 128 -> 3 (SiLU), BF16 weights, FP32 accumulation,
+ * predicts the CG start from the node's last three velocities (the
+ * displacement increments divided by dt) and its rest stiffness diagonal:
+ *   s = |v^n| + |v^n-1| + |v^n-2|,  features = (v^n, v^n-1, v^n-2) / s (9),
+ *   log(1 + cK K_ii,dd / (cM m_i)) / 4 (3),  cM = 1 + alpha dt, cK = dt^2 + beta dt;
+ *   start = v^n + y0 (v^n - v^n-1) + y1 (v^n-1 - v^n-2) + y2 v^n
+ * (y = network output; y = 0 is the plain start, a new model's y = (2, -1, 0)
+ * is quadratic extrapolation).  CG then runs to the same stopping rule as
+ * with the plain start.
+ * Parameters are FP32 masters, in the order W1[128][12], b1[128],
+ * W2[128][128], b2, W3[128][128], b3, W4[3][128], b4[3] (row-major
+ * [out][in]); kernels use BF16 copies refreshed after every change.
+ * Samples are (features[12], target[4] = u*(3), loss weight); a solver with
+ * recording on appends one per free node and step: u* = (v^n+1 - v^n) / s,
+ * weight s^2.  With u = (start - v^n) / s = y0 D1 + y1 D2 + y2 f0..2
+ * (D1 = f0..2 - f3..5, D2 = f3..5 - f6..8), the per-sample loss is
+ *   L = sum_c (w_c / mean w) |u_c - u*_c|^2 / (2 n).
+ * A model and the solvers it is attached to must be used from one thread. */
+typedef struct otfem_model *otfem_model_handle;
+enum { OTFEM_MLP_INPUTS = 12, OTFEM_MLP_OUTPUTS = 3, OTFEM_MLP_PARAMS = 35075 };
+
+/* New model on the current device: hidden layers uniform(+-1/sqrt(fan_in)),
+ * output weights zero, output bias (2, -1, 0). */
+otfem_status otfem_model_create(uint64_t seed, otfem_model_handle *out);
+otfem_status otfem_model_destroy(otfem_model_handle m);
+const char *otfem_model_last_error(otfem_model_handle m);
+otfem_status otfem_model_get_params(otfem_model_handle m, float *theta);
+/* Replaces the parameters and clears the Adam state.  Non-finite values are
+ * accepted: a network that produces non-finite starts falls back to v^n. */
+otfem_status otfem_model_set_params(otfem_model_handle m, const float *theta);
+/* Network outputs for n feature rows ([n][12] -> [n][3]) on the device. */
+otfem_status otfem_model_forward(otfem_model_handle m, const float *features, int64_t n,
+                                 float *out);
+/* Loss and its gradient ([OTFEM_MLP_PARAMS]) over n samples, computed by the
+ * device's training kernels (grad may be NULL). */
+otfem_status otfem_model_gradient(otfem_model_handle m, const float *features,
+                                  const float *targets, int64_t n, double *loss, float *grad);
+/* Device sample store: capacity, count, clear, read back. */
+otfem_status otfem_model_reserve(otfem_model_handle m, int64_t capacity);
+otfem_status otfem_model_samples(otfem_model_handle m, int64_t *count, int64_t *capacity);
+otfem_status otfem_model_clear_samples(otfem_model_handle m);
+otfem_status otfem_model_get_samples(otfem_model_handle m, int64_t offset, int64_t n,
+                                     float *features, float *targets);
+/* Append n caller-made samples (features [n][12], targets [n][4]). */
+otfem_status otfem_model_add_samples(otfem_model_handle m, const float *features,
+                                     const float *targets, int64_t n);
+/* Adam (beta 0.9, 0.999, eps 1e-8) on the per-sample loss for `steps`
+ * minibatches of `batch` samples (a multiple of 128) drawn uniformly from the
+ * store; loss[steps] (may be NULL) receives each minibatch's loss before its
+ * update. */
+otfem_status otfem_model_train(otfem_model_handle m, int steps, int batch, double lr,
+                               uint64_t seed, float *loss);
+
+/* Adam on the residual loss, for samples recorded by solver h (the store
+ * must hold whole steps of h's free nodes, nothing else).  Each minibatch is
+ * `steps_per_batch` recorded steps; with e = v_start - v^{n+1} and
+ * e0 = v^n - v^{n+1} over the free nodes of a step,
+ *   L = sum over the minibatch's steps of |A e|^2 / the same sum of |A e0|^2,
+ * A = cM M + cK K_rest (lumped mass, rest stiffness; cM, cK from dt and
+ * params, NULL = the relative-residual settings).  The ratio is the squared
+ * residual of the learned start relative to the plain start's, with the rest
+ * operator in place of the step's warped one. */
+otfem_status otfem_model_train_residual(otfem_model_handle m, otfem_handle h, double dt,
+                                        const otfem_params *params, int steps,
+                                        int steps_per_batch, double lr, uint64_t seed,
+                                        float *loss);
+/* That loss and its gradient over the given recorded steps (no update). */
+otfem_status otfem_model_residual_gradient(otfem_model_handle m, otfem_handle h, double dt,
+                                           const otfem_params *params, const int32_t *steps,
+                                           int num_steps, double *loss, float *grad);
+
+/* Attach a model to a solver (NULL detaches).  While attached, every step
+ * keeps the velocity history the features need (cleared by otfem_init and by
+ * otfem_set_state with velocities).  Steps with params->warm_start == OTFEM_WARM_LEARNED
+ * start CG from the network's prediction; a node whose prediction is not
+ * finite starts from v^n and sets bit 2 of otfem_stats.flags. */
+otfem_status otfem_set_warmstart_model(otfem_handle h, otfem_model_handle m);
+/* While on, each step appends one sample per free node (in the caller's node
+ * order) to the attached model's store (otfem_model_reserve first; a full
+ * store fails the step). */
+otfem_status otfem_set_recording(otfem_handle h, int on);
 
 #ifdef __cplusplus
 }

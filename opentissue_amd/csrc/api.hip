@@ -14,6 +14,7 @@
 
 #include "kernels.h"
 #include "layout.h"
+#include "mlp.h"
 #include "otfem/otfem.h"
 
 using namespace otfem;
@@ -70,6 +71,16 @@ struct otfem_solver {
   // per-step event triples of asynchronous timed steps, resolved at sync
   std::vector<hipEvent_t> pool;
   size_t pool_used = 0;
+  // learned warm start
+  otfem_model *model = nullptr;
+  bool recording = false;
+  bool ws_suspended = false;     // autotune trials run the plain start
+  std::vector<float> kd_host;    // [4N] rest stiffness diagonal, internal order
+  int64_t n_free = 0;
+  double *vw = nullptr;          // [3N] CG start of learned steps
+  float4 *hist1 = nullptr, *hist2 = nullptr, *kd = nullptr;
+  int32_t *free_rank = nullptr;
+  int *ws_bad = nullptr;
 };
 
 namespace {
@@ -109,6 +120,8 @@ void free_device(otfem_solver *h) {
   dfree(h->Re); dfree(h->plastic); dfree(h->plastic_param);
   dfree(h->body_tile_begin); dfree(h->body); dfree(h->part_bb); dfree(h->part_rr);
   dfree(h->part4); dfree(h->done_count); dfree(h->iter_acc);
+  dfree(h->vw); dfree(h->hist1); dfree(h->hist2); dfree(h->kd); dfree(h->free_rank);
+  dfree(h->ws_bad);
   if (h->g_elem) (void)hipGraphExecDestroy(h->g_elem);
   if (h->g_cg) (void)hipGraphExecDestroy(h->g_cg);
   h->g_elem = h->g_cg = nullptr;
@@ -143,6 +156,38 @@ void to_caller3(const Layout &L, const std::vector<double> &src, double *dst) {
     dst[3 * u + 1] = src[3 * g + 1];
     dst[3 * u + 2] = src[3 * g + 2];
   }
+}
+
+// Free-node ranks (internal order) for the sample store.
+otfem_status ws_upload_free(otfem_solver *h) {
+  const Layout &L = h->L;
+  // a step's samples follow the caller's order of its free nodes
+  std::vector<int32_t> by_caller(L.N), rank(L.N);
+  int64_t n = 0;
+  for (int64_t u = 0; u < L.N; ++u) by_caller[u] = h->fixed_host[u] ? -1 : (int32_t)n++;
+  for (int64_t g = 0; g < L.N; ++g) rank[g] = by_caller[L.node_of_internal[g]];
+  h->n_free = n;
+  HIPCHK(h, hipMemcpyAsync(h->free_rank, rank.data(), L.N * sizeof(int32_t), hipMemcpyHostToDevice,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return OTFEM_OK;
+}
+
+// Velocity history of the learned start: zero = at rest before this state.
+otfem_status ws_reset_history(otfem_solver *h) {
+  if (!h->hist1) return OTFEM_OK;
+  HIPCHK(h, hipMemsetAsync(h->hist1, 0, h->L.N * sizeof(float4), h->stream));
+  HIPCHK(h, hipMemsetAsync(h->hist2, 0, h->L.N * sizeof(float4), h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return OTFEM_OK;
+}
+
+otfem_status ws_upload_kd(otfem_solver *h) {
+  if (!h->kd || h->kd_host.empty()) return OTFEM_OK;
+  HIPCHK(h, hipMemcpyAsync(h->kd, h->kd_host.data(), h->L.N * sizeof(float4), hipMemcpyHostToDevice,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return OTFEM_OK;
 }
 
 otfem_status ensure_device(otfem_solver *h) {
@@ -314,6 +359,31 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
     double amount = h->density[t] * ed[t].V * .25;
     for (int k = 0; k < 4; ++k) h->mass_host[h->tets[4 * t + k]] += amount;
   }
+  // Diagonal of the rest stiffness K_ii (R = I): per tet and corner k with
+  // gradient b, (Ke_kk)_dd = D0 b_d^2 + D2 (|b|^2 - b_d^2) (fem_compute_ke.h:78-99,
+  // with D = V (d16, d17, d18)); feature of the learned warm start.
+  {
+    std::vector<double> kdc(3 * N, 0.0);
+    for (int64_t t = 0; t < T; ++t) {
+      const double *gr = ed[t].grad;
+      double b[4][3];
+      for (int d = 0; d < 3; ++d) {
+        b[1][d] = gr[d];
+        b[2][d] = gr[3 + d];
+        b[3][d] = gr[6 + d];
+        b[0][d] = -(b[1][d] + b[2][d] + b[3][d]);
+      }
+      for (int k = 0; k < 4; ++k) {
+        const double n2 = b[k][0] * b[k][0] + b[k][1] * b[k][1] + b[k][2] * b[k][2];
+        for (int d = 0; d < 3; ++d)
+          kdc[3 * (int64_t)h->tets[4 * t + k] + d] +=
+              ed[t].vd[0] * b[k][d] * b[k][d] + ed[t].vd[2] * (n2 - b[k][d] * b[k][d]);
+      }
+    }
+    h->kd_host.assign(4 * N, 0.0f);
+    for (int64_t g = 0; g < N; ++g)
+      for (int d = 0; d < 3; ++d) h->kd_host[4 * g + d] = (float)kdc[3 * L.node_of_internal[g] + d];
+  }
   std::vector<double> rest(9 * S), vd(3 * S);
   for (int64_t s = 0; s < S; ++s) {
     int64_t t = L.slot_tet[s] & 0x7fffffff;
@@ -349,12 +419,14 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
   h->graph_key.clear();
   h->initialized = true;
   h->rotations_valid = false;
-  return OTFEM_OK;
+  st = ws_upload_kd(h);
+  if (st != OTFEM_OK) return st;
+  return ws_reset_history(h);
 }
 
 StepArgs make_args(otfem_solver *h, double dt, int warp, const otfem_params &P) {
   StepArgs a{};
-  a.x = h->x; a.x0 = h->x0; a.v = h->v; a.fext = h->fext; a.mass = h->mass; a.fixed = h->fixed;
+  a.x = h->x; a.x0 = h->x0; a.v = h->v; a.vw = h->v; a.fext = h->fext; a.mass = h->mass; a.fixed = h->fixed;
   a.rhs = h->rhs; a.r = h->r; a.p = h->p; a.y = h->y;
   a.tiles = h->tiles; a.halo = h->halo; a.conn = h->conn; a.slot_tet = h->slot_tet;
   a.rest64 = h->rest64; a.vd64 = h->vd64; a.bt4 = h->bt4; a.bt8 = h->bt8;
@@ -403,6 +475,11 @@ otfem_status sync_and_check(otfem_solver *h, otfem_stats *stats) {
   h->pending = false;
   otfem_status ts = resolve_timing(h);
   if (ts != OTFEM_OK) return ts;
+  int ws_bad = 0;
+  if (h->ws_bad) {
+    HIPCHK(h, hipMemcpy(&ws_bad, h->ws_bad, sizeof(int), hipMemcpyDeviceToHost));
+    if (ws_bad) HIPCHK(h, hipMemset(h->ws_bad, 0, sizeof(int)));
+  }
   h->last_body.resize(h->L.B);
   HIPCHK(h, hipMemcpy(h->last_body.data(), h->body, h->L.B * sizeof(BodyState),
                       hipMemcpyDeviceToHost));
@@ -419,6 +496,7 @@ otfem_status sync_and_check(otfem_solver *h, otfem_stats *stats) {
     stats->rr = h->last_body[0].rr;
     stats->bb = h->last_body[0].bb;
     stats->elapsed_ms = h->last_elem_ms + h->last_cg_ms;
+    if (ws_bad) stats->flags |= 2;
   }
   for (const BodyState &b : h->last_body)
     if (b.flags & 1) ret = fail(h, OTFEM_NONFINITE, "a CG scalar became non-finite");
@@ -585,6 +663,7 @@ otfem_status otfem_set_state(otfem_handle h, const double *x, const double *v) {
     HIPCHK(h, hipMemcpyAsync(h->v, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice,
                              h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    return ws_reset_history(h);  // new velocities: the old history no longer leads here
   }
   return OTFEM_OK;
 }
@@ -617,6 +696,7 @@ otfem_status otfem_set_fixed(otfem_handle h, const unsigned char *fixed) {
   for (int64_t g = 0; g < h->L.N; ++g) fx[g] = h->fixed_host[h->L.node_of_internal[g]];
   HIPCHK(h, hipMemcpyAsync(h->fixed, fx.data(), fx.size(), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (h->free_rank) return ws_upload_free(h);
   return OTFEM_OK;
 }
 
@@ -671,11 +751,42 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
       (P.tol_mode != 0 && P.tol_mode != 1) || (P.rotation_mode != 0 && P.rotation_mode != 1) ||
       !std::isfinite(P.alpha) || !std::isfinite(P.beta))
     return fail(h, OTFEM_INVALID_ARG, "bad solver parameters");
-  if (P.warm_start != OTFEM_WARM_PREVIOUS_VELOCITY)
-    return fail(h, OTFEM_INVALID_ARG, "warm start model not loaded");
+  if (P.warm_start != OTFEM_WARM_PREVIOUS_VELOCITY && P.warm_start != OTFEM_WARM_LEARNED)
+    return fail(h, OTFEM_INVALID_ARG, "unknown warm start");
+  const bool ws_on = h->model && !h->ws_suspended;
+  const bool learned = P.warm_start == OTFEM_WARM_LEARNED;
+  if (learned && !ws_on) return fail(h, OTFEM_STATE, "no warm-start model attached");
+  WsArgs w{};
+  ModelView mv{};
+  if (ws_on) {
+    model_view(h->model, &mv);
+    if (h->recording && mv.capacity - mv.count < h->n_free)
+      return fail(h, OTFEM_STATE, "sample store full (otfem_model_reserve)");
+    w.v = h->v;
+    w.vw = h->vw;
+    w.hist1 = h->hist1;
+    w.hist2 = h->hist2;
+    w.kd = h->kd;
+    w.mass = h->mass;
+    w.fixed = h->fixed;
+    w.free_rank = h->free_rank;
+    w.N = h->L.N;
+    w.cM = 1.0 + P.alpha * dt;
+    w.cK = dt * dt + P.beta * dt;
+    w.predict = learned ? 1 : 0;
+    w.record = h->recording ? 1 : 0;
+    w.rec_base = mv.count;
+    w.rec_X = mv.X;
+    w.rec_S = mv.S;
+    w.rec_T = mv.T;
+    w.wf = mv.wf;
+    w.theta = mv.theta;
+    w.bad = h->ws_bad;
+  }
   if (h->lc.spmv_mode == SPMV_F64_STORE64 && !h->bt8)
     HIPCHK(h, dalloc(&h->bt8, 3 * (size_t)h->L.slot_tet.size()));
   StepArgs a = make_args(h, dt, use_stiffness_warping, P);
+  if (learned) a.vw = h->vw;
   hipStream_t s = h->stream;
   const bool timed = (stats != nullptr) || h->timing;
   hipEvent_t e[3] = {nullptr, nullptr, nullptr};
@@ -689,6 +800,7 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
     h->pool_used += 3;
     HIPCHK(h, hipEventRecord(e[0], s));
   }
+  if (ws_on) HIPCHK(h, enqueue_ws_predict(w, s));
   if (h->use_graphs) {
     // the graphs bake in every kernel argument: rebuild when any changes
     std::vector<unsigned char> key(sizeof(StepArgs) + sizeof(LaunchConfig));
@@ -723,6 +835,10 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
     HIPCHK(h, enqueue_element(a, h->lds_elem, s));
     if (timed) HIPCHK(h, hipEventRecord(e[1], s));
     HIPCHK(h, enqueue_cg(a, h->lc, h->lds_cg, s));
+  }
+  if (ws_on && h->recording) {
+    HIPCHK(h, enqueue_ws_target(w, s));
+    model_commit(h->model, h->n_free);
   }
   if (timed) HIPCHK(h, hipEventRecord(e[2], s));
   h->rotations_valid = (P.store_rotations != 0) && use_stiffness_warping;
@@ -810,6 +926,15 @@ otfem_status otfem_autotune(otfem_handle h, double dt, int use_stiffness_warping
     if ((e = hipMemcpyAsync(restore ? h->plastic : spl, restore ? spl : h->plastic, 6 * T * sizeof(double), k, s))) return e;
     return hipStreamSynchronize(s);
   };
+  // trials use the plain CG start and leave the learned-start history alone
+  otfem_params P;
+  if (params)
+    P = *params;
+  else
+    otfem_default_params(&P, 0);
+  P.warm_start = OTFEM_WARM_PREVIOUS_VELOCITY;
+  params = &P;
+  h->ws_suspended = true;
   otfem_status ret = OTFEM_OK;
   const int blocks[3] = {256, 512, 1024};
   double best = 1e300;
@@ -836,6 +961,7 @@ otfem_status otfem_autotune(otfem_handle h, double dt, int use_stiffness_warping
     }
   }
   h->timing = timing_was;
+  h->ws_suspended = false;
   h->lc.cg_block = best_b;
   dfree(sx); dfree(sv); dfree(sf); dfree(spl);
   if (best_block) *best_block = best_b;
@@ -956,6 +1082,151 @@ otfem_status otfem_layout_info(otfem_handle h, int64_t *num_tiles, int64_t *num_
   if (num_halo) *num_halo = (int64_t)h->L.halo.size();
   if (max_tile_nodes) *max_tile_nodes = h->L.max_tile_nodes;
   return OTFEM_OK;
+}
+
+otfem_status otfem_set_warmstart_model(otfem_handle h, otfem_model_handle m) {
+  if (!h) return OTFEM_INVALID_ARG;
+  if (!m) {
+    h->model = nullptr;
+    h->recording = false;
+    return OTFEM_OK;
+  }
+  otfem_status st = ensure_device(h);
+  if (st) return st;
+  ModelView mv{};
+  model_view(m, &mv);
+  if (mv.device != h->device) return fail(h, OTFEM_INVALID_ARG, "model and solver are on different devices");
+  const int64_t N = h->L.N;
+  if (!h->vw) {
+    HIPCHK(h, dalloc(&h->vw, 3 * N));
+    HIPCHK(h, dalloc(&h->hist1, N));
+    HIPCHK(h, dalloc(&h->hist2, N));
+    HIPCHK(h, dalloc(&h->kd, N));
+    HIPCHK(h, dalloc(&h->free_rank, N));
+    HIPCHK(h, dalloc(&h->ws_bad, 1));
+    HIPCHK(h, hipMemsetAsync(h->kd, 0, N * sizeof(float4), h->stream));
+    HIPCHK(h, hipMemsetAsync(h->ws_bad, 0, sizeof(int), h->stream));
+    st = ws_upload_kd(h);
+    if (st) return st;
+  }
+  st = ws_upload_free(h);
+  if (st) return st;
+  st = ws_reset_history(h);
+  if (st) return st;
+  h->model = m;
+  return OTFEM_OK;
+}
+
+otfem_status otfem_set_recording(otfem_handle h, int on) {
+  if (!h) return OTFEM_INVALID_ARG;
+  if (on && !h->model) return fail(h, OTFEM_STATE, "attach a model before recording");
+  h->recording = on != 0;
+  return OTFEM_OK;
+}
+
+// A = cM M + cK K_rest over the free nodes (caller order), K_rest from the
+// rest element stiffness Ke_ab = V [d17 g_a g_b^T + d18 g_b g_a^T + d18 (g_a.g_b) I]
+// (fem_compute_ke.h:78-99 with d16 = d17 + 2 d18), fixed rows and columns dropped.
+static otfem_status residual_operator(otfem_solver *h, double dt, const otfem_params *params,
+                                      ResidualOp &op) {
+  if (!h->initialized) return fail(h, OTFEM_STATE, "call otfem_init first");
+  if (!(dt > 0)) return fail(h, OTFEM_INVALID_ARG, "dt must be positive");
+  otfem_params P;
+  if (params)
+    P = *params;
+  else
+    otfem_default_params(&P, 1);
+  const double cM = 1.0 + P.alpha * dt, cK = dt * dt + P.beta * dt;
+  const int64_t N = h->L.N, T = h->L.T;
+  std::vector<int32_t> rank(N);
+  int64_t n = 0;
+  for (int64_t u = 0; u < N; ++u) rank[u] = h->fixed_host[u] ? -1 : (int32_t)n++;
+  std::vector<std::vector<std::pair<int32_t, std::array<double, 9>>>> rows(n);
+  auto add_block = [&](int32_t r, int32_t c, const double *b) {
+    for (auto &e : rows[r])
+      if (e.first == c) {
+        for (int q = 0; q < 9; ++q) e.second[q] += b[q];
+        return;
+      }
+    std::array<double, 9> a;
+    for (int q = 0; q < 9; ++q) a[q] = b[q];
+    rows[r].emplace_back(c, a);
+  };
+  const double *x0 = h->x0_host.data();
+  for (int64_t t = 0; t < T; ++t) {
+    const int32_t *tt = &h->tets[4 * t];
+    const ElemData ed = element_data(x0 + 3 * tt[0], x0 + 3 * tt[1], x0 + 3 * tt[2],
+                                     x0 + 3 * tt[3], h->young[t], h->poisson[t]);
+    double g[4][3];
+    for (int d = 0; d < 3; ++d) {
+      g[1][d] = ed.grad[d];
+      g[2][d] = ed.grad[3 + d];
+      g[3][d] = ed.grad[6 + d];
+      g[0][d] = -(g[1][d] + g[2][d] + g[3][d]);
+    }
+    for (int a = 0; a < 4; ++a) {
+      const int32_t ra = rank[tt[a]];
+      if (ra < 0) continue;
+      for (int b = 0; b < 4; ++b) {
+        const int32_t rb = rank[tt[b]];
+        if (rb < 0) continue;
+        const double dot = g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2];
+        double blk[9];
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c)
+            blk[3 * r + c] = cK * (ed.vd[1] * g[a][r] * g[b][c] + ed.vd[2] * g[b][r] * g[a][c] +
+                                   (r == c ? ed.vd[2] * dot : 0.0));
+        add_block(ra, rb, blk);
+      }
+    }
+  }
+  for (int64_t u = 0; u < N; ++u) {
+    if (rank[u] < 0) continue;
+    const double mm = cM * h->mass_host[u];
+    const double blk[9] = {mm, 0, 0, 0, mm, 0, 0, 0, mm};
+    add_block(rank[u], rank[u], blk);
+  }
+  op.n = n;
+  op.row_ptr.assign(n + 1, 0);
+  op.cols.clear();
+  op.blocks.clear();
+  for (int64_t r = 0; r < n; ++r) {
+    std::sort(rows[r].begin(), rows[r].end(),
+              [](const auto &a, const auto &b) { return a.first < b.first; });
+    for (auto &e : rows[r]) {
+      op.cols.push_back(e.first);
+      for (int q = 0; q < 9; ++q) op.blocks.push_back((float)e.second[q]);
+    }
+    op.row_ptr[r + 1] = (int64_t)op.cols.size();
+  }
+  return OTFEM_OK;
+}
+
+otfem_status otfem_model_train_residual(otfem_model_handle m, otfem_handle h, double dt,
+                                        const otfem_params *params, int steps,
+                                        int steps_per_batch, double lr, uint64_t seed,
+                                        float *loss) {
+  if (!m || !h) return OTFEM_INVALID_ARG;
+  ResidualOp op;
+  otfem_status st = residual_operator(h, dt, params, op);
+  if (st) return st;
+  std::string err;
+  st = (otfem_status)model_train_residual(m, op, steps, steps_per_batch, lr, seed, loss, &err);
+  if (st) h->err = err;
+  return st;
+}
+
+otfem_status otfem_model_residual_gradient(otfem_model_handle m, otfem_handle h, double dt,
+                                           const otfem_params *params, const int32_t *steps,
+                                           int num_steps, double *loss, float *grad) {
+  if (!m || !h || !steps || num_steps <= 0) return OTFEM_INVALID_ARG;
+  ResidualOp op;
+  otfem_status st = residual_operator(h, dt, params, op);
+  if (st) return st;
+  std::string err;
+  st = (otfem_status)model_residual(m, op, 1, num_steps, 1.0, 0, nullptr, steps, grad, loss, &err);
+  if (st) h->err = err;
+  return st;
 }
 
 }  // extern "C"

@@ -29,6 +29,7 @@ struct StepArgs {
   double *x;
   const double *x0;
   double *v;
+  const double *vw;      // CG start (== v for the plain start, else a separate buffer)
   double *fext;
   const double *mass;
   const uint8_t *fixed;

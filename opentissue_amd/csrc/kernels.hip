@@ -258,7 +258,7 @@ __global__ __launch_bounds__(kElemBlock) void k_element(StepArgs A) {
     int g = l < no ? ti.node_begin + l : A.halo[ti.halo_begin + l - no];
     for (int d = 0; d < 3; ++d) {
       su[3 * l + d] = A.x[3 * (int64_t)g + d] - A.x0[3 * (int64_t)g + d];
-      sv[3 * l + d] = A.v[3 * (int64_t)g + d];
+      sv[3 * l + d] = A.vw[3 * (int64_t)g + d];
     }
     if (l < no) {
       for (int d = 0; d < 3; ++d) {
@@ -458,11 +458,18 @@ __global__ __launch_bounds__(kElemBlock) void k_element(StepArgs A) {
     }
     const double m = A.mass[g];
     for (int d = 0; d < 3; ++d) {
-      double vel = sv[3 * l + d];
+      // b uses v^n; the residual uses the CG start (fem_conjugate_gradients.h:52-66).
+      // One code path for both starts: v^n is read from memory and the start
+      // is always written back as CG's iterate (with the plain start vw == v
+      // and the store rewrites the value just read), so the arithmetic does
+      // not depend on where the start came from.
+      const double vs = sv[3 * l + d];
+      const double vel = A.v[3 * g + d];
       double bd = (sf[3 * l + d] + fx[d]) * A.dt + vel * m;
-      double rd = bd - (A.cM * m * vel + A.cK * skv[3 * l + d]);
+      double rd = bd - (A.cM * m * vs + A.cK * skv[3 * l + d]);
       A.rhs[3 * g + d] = bd;
       A.r[3 * g + d] = rd;
+      A.v[3 * g + d] = vs;
       bb += bd * bd;
       rr += rd * rd;
     }

@@ -83,7 +83,12 @@ class MeshDesc(C.Structure):
 
 # Every symbol of include/otfem/otfem.h, with (restype, argtypes).
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 _H = C.c_void_p
+_M = C.c_void_p
+
+WARM_PREVIOUS_VELOCITY, WARM_LEARNED = 0, 1
+MLP_INPUTS, MLP_OUTPUTS, MLP_PARAMS = 12, 3, 35075
 SIGNATURES = {
     "otfem_version": (C.c_char_p, []),
     "otfem_status_string": (C.c_char_p, [C.c_int]),
@@ -116,6 +121,25 @@ SIGNATURES = {
     "otfem_export_K": (C.c_int, [_H, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                  C.POINTER(C.c_int32), _dp]),
     "otfem_layout_info": (C.c_int, [_H] + [C.POINTER(C.c_int64)] * 4),
+    "otfem_model_create": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "otfem_model_destroy": (C.c_int, [_M]),
+    "otfem_model_last_error": (C.c_char_p, [_M]),
+    "otfem_model_get_params": (C.c_int, [_M, _fp]),
+    "otfem_model_set_params": (C.c_int, [_M, _fp]),
+    "otfem_model_forward": (C.c_int, [_M, _fp, C.c_int64, _fp]),
+    "otfem_model_gradient": (C.c_int, [_M, _fp, _fp, C.c_int64, _dp, _fp]),
+    "otfem_model_reserve": (C.c_int, [_M, C.c_int64]),
+    "otfem_model_samples": (C.c_int, [_M, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "otfem_model_clear_samples": (C.c_int, [_M]),
+    "otfem_model_get_samples": (C.c_int, [_M, C.c_int64, C.c_int64, _fp, _fp]),
+    "otfem_model_add_samples": (C.c_int, [_M, _fp, _fp, C.c_int64]),
+    "otfem_model_train": (C.c_int, [_M, C.c_int, C.c_int, C.c_double, C.c_uint64, _fp]),
+    "otfem_model_train_residual": (C.c_int, [_M, _H, C.c_double, C.POINTER(Params), C.c_int, C.c_int,
+                                             C.c_double, C.c_uint64, _fp]),
+    "otfem_model_residual_gradient": (C.c_int, [_M, _H, C.c_double, C.POINTER(Params),
+                                                C.POINTER(C.c_int32), C.c_int, _dp, _fp]),
+    "otfem_set_warmstart_model": (C.c_int, [_H, _M]),
+    "otfem_set_recording": (C.c_int, [_H, C.c_int]),
 }
 
 
@@ -174,6 +198,7 @@ class Solver:
         mc = None if model_coords is None else _f64(model_coords, (self.N, 3))
         vel = None if velocity is None else _f64(velocity, (self.N, 3))
         fx = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.uint8)
+        self.n_free = self.N if fx is None else int((fx == 0).sum())
         d = MeshDesc()
         d.num_nodes, d.num_tets = self.N, self.T
         d.coord = _dptr(coords)
@@ -248,6 +273,7 @@ class Solver:
 
     def set_fixed(self, fixed):
         fx = np.ascontiguousarray(fixed, dtype=np.uint8)
+        self.n_free = int((fx == 0).sum())
         self._check(self.lib.otfem_set_fixed(self.h, fx.ctypes.data_as(C.POINTER(C.c_ubyte))))
 
     def mass(self):
@@ -339,6 +365,134 @@ class Solver:
         self._check(self.lib.otfem_layout_info(self.h, *[C.byref(v) for v in vals]))
         return dict(tiles=vals[0].value, slots=vals[1].value, halo=vals[2].value,
                     max_tile_nodes=vals[3].value)
+
+    # learned warm start
+    def set_warmstart_model(self, model):
+        """Attach a :class:`Model` (None detaches).  Keeps a reference."""
+        self._check(self.lib.otfem_set_warmstart_model(self.h, model.h if model is not None else None))
+        self._model = model
+
+    def set_recording(self, on=True):
+        self._check(self.lib.otfem_set_recording(self.h, 1 if on else 0))
+
+
+def _f32(a, shape):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(shape))
+
+
+class Model:
+    """Per-node MLP of the learned CG warm start (``otfem_model_*``).
+
+    Parameters are FP32 masters in the order W1[128,12], b1, W2[128,128], b2,
+    W3[128,128], b3, W4[3,128], b4; the device kernels run BF16 copies."""
+
+    def __init__(self, seed=0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        st = self.lib.otfem_model_create(seed, C.byref(h))
+        if st != OK:
+            raise OtfemError(st, "otfem_model_create failed")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.otfem_model_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != OK:
+            msg = self.lib.otfem_model_last_error(self.h)
+            raise OtfemError(st, msg.decode() if msg else "")
+        return st
+
+    def params(self):
+        th = np.empty(MLP_PARAMS, dtype=np.float32)
+        self._check(self.lib.otfem_model_get_params(self.h, th.ctypes.data_as(_fp)))
+        return th
+
+    def set_params(self, theta):
+        th = _f32(theta, (MLP_PARAMS,))
+        self._check(self.lib.otfem_model_set_params(self.h, th.ctypes.data_as(_fp)))
+
+    def forward(self, features):
+        X = _f32(features, (-1, MLP_INPUTS))
+        Y = np.empty((X.shape[0], MLP_OUTPUTS), dtype=np.float32)
+        self._check(self.lib.otfem_model_forward(self.h, X.ctypes.data_as(_fp), X.shape[0],
+                                                 Y.ctypes.data_as(_fp)))
+        return Y
+
+    def gradient(self, features, targets):
+        """(loss, grad[MLP_PARAMS]) from the device training kernels."""
+        X = _f32(features, (-1, MLP_INPUTS))
+        T = _f32(targets, (X.shape[0], 4))
+        g = np.empty(MLP_PARAMS, dtype=np.float32)
+        loss = C.c_double()
+        self._check(self.lib.otfem_model_gradient(self.h, X.ctypes.data_as(_fp), T.ctypes.data_as(_fp),
+                                                  X.shape[0], C.byref(loss), g.ctypes.data_as(_fp)))
+        return loss.value, g
+
+    def reserve(self, capacity):
+        self._check(self.lib.otfem_model_reserve(self.h, int(capacity)))
+
+    def num_samples(self):
+        n = C.c_int64()
+        self._check(self.lib.otfem_model_samples(self.h, C.byref(n), None))
+        return n.value
+
+    def clear_samples(self):
+        self._check(self.lib.otfem_model_clear_samples(self.h))
+
+    def samples(self, offset=0, n=None):
+        if n is None:
+            n = self.num_samples() - offset
+        X = np.empty((n, MLP_INPUTS), dtype=np.float32)
+        T = np.empty((n, 4), dtype=np.float32)
+        self._check(self.lib.otfem_model_get_samples(self.h, offset, n, X.ctypes.data_as(_fp),
+                                                     T.ctypes.data_as(_fp)))
+        return X, T
+
+    def add_samples(self, features, targets):
+        X = _f32(features, (-1, MLP_INPUTS))
+        T = _f32(targets, (X.shape[0], 4))
+        self._check(self.lib.otfem_model_add_samples(self.h, X.ctypes.data_as(_fp), T.ctypes.data_as(_fp),
+                                                     X.shape[0]))
+
+    def train_residual(self, solver, dt, params=None, steps=1000, steps_per_batch=None, lr=1e-3,
+                       seed=0):
+        """Adam on the residual loss over steps recorded by `solver`; the
+        default minibatch is the whole steps closest to 65,536 nodes."""
+        if steps_per_batch is None:
+            steps_per_batch = max(1, round(65536 / max(1, int(solver.n_free))))
+        P = config_params() if params is None else params
+        loss = np.empty(max(steps, 1), dtype=np.float32)
+        st = self.lib.otfem_model_train_residual(self.h, solver.h, dt, C.byref(P), steps,
+                                                 steps_per_batch, lr, seed, loss.ctypes.data_as(_fp))
+        if st != OK:
+            raise OtfemError(st, (self.lib.otfem_model_last_error(self.h) or b"").decode())
+        return loss[:steps]
+
+    def residual_gradient(self, solver, dt, steps, params=None):
+        P = config_params() if params is None else params
+        st_idx = np.ascontiguousarray(steps, dtype=np.int32)
+        g = np.empty(MLP_PARAMS, dtype=np.float32)
+        loss = C.c_double()
+        st = self.lib.otfem_model_residual_gradient(self.h, solver.h, dt, C.byref(P),
+                                                    st_idx.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                    len(st_idx), C.byref(loss), g.ctypes.data_as(_fp))
+        if st != OK:
+            raise OtfemError(st, (self.lib.otfem_model_last_error(self.h) or b"").decode())
+        return loss.value, g
+
+    def train(self, steps, batch=65536, lr=1e-3, seed=0):
+        loss = np.empty(max(steps, 1), dtype=np.float32)
+        self._check(self.lib.otfem_model_train(self.h, steps, batch, lr, seed, loss.ctypes.data_as(_fp)))
+        return loss[:steps]
 
 
 # ---------------------------------------------------------------------------
