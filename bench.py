@@ -68,6 +68,8 @@ def build_workload(name):
         return workloads.cantilever()
     if name == "bodies":
         return workloads.bodies(512, 4, 24)
+    if name == "combined":
+        return workloads.combined(2048, 4, 32)
     raise SystemExit(f"unknown config {name}")
 
 
@@ -99,6 +101,40 @@ def run_cpu_baseline(w, params_kw, seconds_budget):
     t = time.perf_counter() - t0
     x, v = o.state()
     return dict(seconds=t, init_seconds=t_init, iterations=st.iterations, x=x, v=v)
+
+
+def run_cpu_baseline_bodies(w, params_kw, seconds_budget, xg):
+    """Bodies stepped one by one (each its own CG, as on the device) with the
+    FP64 oracle from rest until the time budget is spent; also the first-step
+    displacement error of the device against those bodies."""
+    import oracle
+    P = oracle.config_params(**params_kw)
+    no, to = w.body_node_offsets, w.body_tet_offsets
+    secs, tets, its, bodies = 0.0, 0, 0, 0
+    err = {"converged": 0.0, "capped": 0.0}
+    n_capped = 0
+    for b in range(w.num_bodies):
+        ns, ts = slice(no[b], no[b + 1]), slice(to[b], to[b + 1])
+        c = w.coords[ns]
+        o = oracle.Oracle(c, w.tets[ts] - no[b], w.fixed[ns])
+        o.init_materials(w.young[ts], w.poisson[ts], w.density[ts])
+        o.set_external_forces(w.forces[ns])
+        t0 = time.perf_counter()
+        st = o.simulate(w.dt, True, P)
+        secs += time.perf_counter() - t0
+        x, _ = o.state()
+        den = np.linalg.norm(x - c, axis=1).max()
+        key = "converged" if st.converged else "capped"
+        n_capped += 0 if st.converged else 1
+        if den > 0:
+            err[key] = max(err[key], np.linalg.norm(xg[ns] - x, axis=1).max() / den)
+        tets += int(to[b + 1] - to[b])
+        its += st.iterations
+        bodies += 1
+        if secs >= seconds_budget:
+            break
+    return dict(seconds=secs, tets=tets, bodies=bodies, iterations=its, err=err,
+                capped=n_capped)
 
 
 def train_warmstart_model():
@@ -148,7 +184,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="cube64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=60.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="time budget of the multi-body CPU sample")
     ap.add_argument("--tile-nodes", type=int, default=0)
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm start legs")
@@ -289,7 +326,32 @@ def main():
             "ab_uncapped": ab,
         }
 
-    if not args.no_cpu:
+    if not args.no_cpu and w.num_bodies > 1:
+        # bounded sample: bodies of the same workload, one step from rest each
+        s.set_state(w.coords, zeros)
+        s.simulate(w.dt, True, P)
+        xg, _ = s.state()
+        log(f"cpu baseline: bodies of the same workload, one step each, FP64 oracle (1 thread), "
+            f"{args.cpu_seconds:.0f} s budget")
+        cb = run_cpu_baseline_bodies(w, params_kw, args.cpu_seconds, xg)
+        model, ncpu = cpu_info()
+        result["cpu_baseline"] = {
+            "value": cb["tets"] / cb["seconds"],
+            "unit": "tet-steps/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"first {cb['bodies']} bodies of the same {w.name} workload ({cb['tets']} tets), "
+                      f"one implicit step from rest each ({cb['iterations']} CG iterations in all), "
+                      f"FP64 ordered-map restatement of fem::simulate, {cb['seconds']:.1f} s on {model} "
+                      f"({ncpu} host cpus, 1 used)",
+        }
+        result["parity_first_step"] = {
+            "bodies": cb["bodies"],
+            "bodies_capped": cb["capped"],
+            "rel_displacement_error_max_converged_bodies": cb["err"]["converged"],
+            "rel_displacement_error_max_capped_bodies": cb["err"]["capped"],
+        }
+    elif not args.no_cpu:
         # bounded sample: one step from rest of the same workload; the GPU's
         # first step from the same state is the parity check at full size
         s2 = fem.Solver(w.coords, w.tets, w.fixed, body_node_offsets=w.body_node_offsets,

@@ -9,8 +9,9 @@
 //     tile-local indices into the tile's node table [owned..., halo...].
 //   * A tile computes element terms for all its slots but accumulates only
 //     into its owned nodes, so every node sum is complete inside one
-//     workgroup: no global atomics, deterministic results, and per-node
-//     epilogues (dot products, axpys) fuse into the same pass.
+//     workgroup: no global atomics, and per-node epilogues (dot products,
+//     axpys) fuse into the same pass.  The node sums are LDS atomics, so
+//     their rounding depends on wave order (not bitwise reproducible).
 #pragma once
 #include <cstdint>
 #include <string>

@@ -28,6 +28,8 @@ __all__ = [
     "tet_volumes",
     "boundary_faces",
     "jitter_interior",
+    "jitter_bodies",
+    "plane_faces",
     "lumped_mass",
     "gravity_forces",
     "traction_forces",
@@ -101,7 +103,12 @@ def boundary_faces(tets: np.ndarray) -> np.ndarray:
         np.stack([i, k, j], 1),
     ], 0)
     key = np.sort(faces, axis=1)
-    _, inv, counts = np.unique(key, axis=0, return_inverse=True, return_counts=True)
+    n = int(t.max()) + 1 if t.size else 1
+    if n < (1 << 21):  # one int64 per face: a 1-D unique instead of a row unique
+        code = (key[:, 0] << 42) | (key[:, 1] << 21) | key[:, 2]
+        _, inv, counts = np.unique(code, return_inverse=True, return_counts=True)
+    else:
+        _, inv, counts = np.unique(key, axis=0, return_inverse=True, return_counts=True)
     inv = inv.ravel()
     once = counts[inv] == 1
     # keep the reference's emission order: tet by tet, faces jkm, ijm, kim, ikj
@@ -142,6 +149,47 @@ def jitter_interior(coords, tets, h, amplitude, seed, min_volume_frac=0.05, max_
     raise RuntimeError("jitter: resampling did not converge")
 
 
+def jitter_bodies(coords, tets, node_offsets, h, amplitude, seed, min_volume_frac=0.05,
+                  max_rounds=200, chunk=1 << 22):
+    """jitter_interior for a CSR-of-bodies mesh in one pass: each body's
+    bounding-box nodes stay put, interior nodes move uniformly by +/-
+    amplitude*h per axis, and nodes of tets below ``min_volume_frac * h**3``
+    are redrawn until every tet passes (volumes checked in chunks)."""
+    rng = np.random.default_rng(seed)
+    x0 = np.asarray(coords, dtype=np.float64)
+    x = x0.copy()
+    no = np.asarray(node_offsets, dtype=np.int64)
+    counts = np.diff(no)
+    lo = np.repeat(np.minimum.reduceat(x0, no[:-1], axis=0), counts, axis=0)
+    hi = np.repeat(np.maximum.reduceat(x0, no[:-1], axis=0), counts, axis=0)
+    tol = 1e-9 * h
+    interior = np.all((x0 > lo + tol) & (x0 < hi - tol), axis=1)
+    del lo, hi
+    idx = np.nonzero(interior)[0]
+    x[idx] += rng.uniform(-amplitude * h, amplitude * h, size=(idx.size, 3))
+    vmin = min_volume_frac * h ** 3
+    check = np.arange(len(tets))
+    for _ in range(max_rounds):
+        bad = []
+        for a in range(0, len(check), chunk):
+            sel = check[a:a + chunk]
+            V = tet_volumes(x, tets[sel])
+            bad.append(sel[V < vmin])
+        bad = np.concatenate(bad) if bad else np.zeros(0, dtype=np.int64)
+        if bad.size == 0:
+            return x
+        nodes = np.unique(tets[bad].ravel())
+        nodes = nodes[interior[nodes]]
+        if nodes.size == 0:
+            raise RuntimeError("jitter: a boundary-only tetrahedron is below the volume floor")
+        x[nodes] = x0[nodes] + rng.uniform(-amplitude * h, amplitude * h, size=(nodes.size, 3))
+        # only tets touching a redrawn node can change
+        touched = np.zeros(len(x), dtype=bool)
+        touched[nodes] = True
+        check = np.nonzero(touched[tets].any(axis=1))[0]
+    raise RuntimeError("jitter: resampling did not converge")
+
+
 def lumped_mass(coords, tets, density, fixed=None):
     """m_i = sum over incident tets of rho_e V_e / 4 (fem_compute_mass.h:164-171).
     Fixed nodes keep the value the reference would give them before the
@@ -169,12 +217,29 @@ def face_nodes(coords, axis, side, tol=1e-9):
     return np.abs(c - ref) <= tol * span
 
 
+def plane_faces(tets, on):
+    """Tet faces whose three nodes all satisfy ``on``, in boundary_faces'
+    emission order.  When ``on`` marks a bounding plane of the mesh every such
+    face is a boundary face (two tets sharing a face lie on opposite sides of
+    it, and no node lies beyond a bounding plane), so no face matching is
+    needed."""
+    t = np.asarray(tets, dtype=np.int64)
+    hit = on[t]                                   # [T, 4]
+    cand = hit.sum(1) >= 3
+    tc = t[cand]
+    i, j, k, m = tc[:, 0], tc[:, 1], tc[:, 2], tc[:, 3]
+    faces = np.stack([np.stack([j, k, m], 1), np.stack([i, j, m], 1),
+                      np.stack([k, i, m], 1), np.stack([i, k, j], 1)], 1)   # [Tc, 4, 3]
+    faces = faces.reshape(-1, 3)
+    return faces[on[faces].all(axis=1)]
+
+
 def traction_forces(coords, tets, axis, side, traction, faces=None):
     """Constant traction ``traction`` (3-vector, Pa) on the bounding-box face
     (axis, side), converted to nodal forces t*A_tri/3 per boundary triangle."""
-    if faces is None:
-        faces = boundary_faces(tets)
     on = face_nodes(coords, axis, side)
+    if faces is None:
+        faces = plane_faces(tets, on)
     sel = faces[on[faces].all(axis=1)]
     p = coords[sel]
     area = 0.5 * np.linalg.norm(np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]), axis=1)

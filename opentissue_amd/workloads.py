@@ -84,11 +84,13 @@ def bodies(count=512, lo=4, hi=24, h=0.01, jitter=0.2, seed=0, traction=1e3, rho
     dims = rng.integers(lo, hi + 1, size=(count, 3))
     Es = np.exp(rng.uniform(np.log(1e4), np.log(1e6), size=count))
     nus = rng.uniform(0.25, 0.45, size=count)
-    parts, fixed_l, E_l, nu_l, f_l = [], [], [], [], []
+    parts = [M.generate_blocks(int(I), int(J), int(K), h, h, h) for I, J, K in dims]
+    coords0, tets, no, to = M.pack_bodies(parts)
+    coords = M.jitter_bodies(coords0, tets, no, h, jitter, seed * 100003 + 1) if jitter > 0 else coords0
+    fixed_l, f_l = [], []
     for b in range(count):
-        I, J, K = (int(v) for v in dims[b])
-        c0, t = M.generate_blocks(I, J, K, h, h, h)
-        c = M.jitter_interior(c0, t, h, jitter, seed * 100003 + b) if jitter > 0 else c0
+        c = coords[no[b]:no[b + 1]]
+        t = tets[to[b]:to[b + 1]] - no[b]
         fx = M.face_nodes(c, 0, 0).astype(np.uint8)
         if random_traction:
             mag = rng.uniform(*traction_range)
@@ -99,14 +101,22 @@ def bodies(count=512, lo=4, hi=24, h=0.01, jitter=0.2, seed=0, traction=1e3, rho
             tr = traction * np.array([0.0, 0.0, -1.0])
         f = M.traction_forces(c, t, 0, 1, tr)
         f[fx == 1] = 0.0
-        parts.append((c, t))
         fixed_l.append(fx)
-        E_l.append(np.full(t.shape[0], Es[b]))
-        nu_l.append(np.full(t.shape[0], nus[b]))
         f_l.append(f)
-    coords, tets, no, to = M.pack_bodies(parts)
+    bt = np.diff(to)
     T = tets.shape[0]
     return Workload("bodies%d_dims%d-%d" % (count, lo, hi), coords, tets,
-                    np.concatenate(fixed_l), np.concatenate(E_l), np.concatenate(nu_l),
+                    np.concatenate(fixed_l), np.repeat(Es, bt), np.repeat(nus, bt),
                     np.full(T, rho), np.concatenate(f_l, 0), 0.01, 100, no, to,
-                    meta=dict(h=h, seed=seed, dims=dims.tolist()))
+                    meta=dict(h=h, seed=seed, dims=dims.tolist(), jitter=jitter,
+                              random_traction=random_traction))
+
+
+def combined(count=2048, lo=4, hi=32, jitter=0.25, seed=0):
+    """Uneven bodies with per-axis dims uniform in [lo, hi] cubes, jitter
+    +/-0.25 h (with resampling), per-body materials and a per-body random
+    traction (0.5e3 .. 2e3 Pa, random direction) on the face opposite the
+    fixed one; dt 0.01, 100 steps."""
+    w = bodies(count, lo, hi, jitter=jitter, seed=seed, random_traction=True)
+    w.name = "combined%d_dims%d-%d" % (count, lo, hi)
+    return w
