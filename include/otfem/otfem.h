@@ -150,6 +150,10 @@ otfem_status otfem_get_timing(otfem_handle h, double *element_ms, double *cg_ms,
  * (reset != 0 clears after reading; totals may be NULL).  No reference
  * counterpart; used to count algorithmic bytes of asynchronous steps. */
 otfem_status otfem_iteration_totals(otfem_handle h, int64_t *totals, int reset);
+/* Per body, the number of steps since the last reset whose CG stopped at the
+ * iteration cap without meeting its stopping rule.  No reference counterpart
+ * (the reference reports no convergence at all). */
+otfem_status otfem_unconverged_totals(otfem_handle h, int64_t *totals, int reset);
 /* Kernel configuration: threads per tile in the CG kernels (256/512/1024),
  * SpMV precision inside CG (OTFEM_SPMV_*: FP32 arithmetic with FP32 or FP64
  * accumulation, FP64 arithmetic on FP32-stored element data (default), or FP64

@@ -55,12 +55,16 @@ struct otfem_solver {
   BodyState *body = nullptr;
   double *part_bb = nullptr, *part_rr = nullptr, *part4 = nullptr;
   int *done_count = nullptr;
-  unsigned long long *iter_acc = nullptr;
+  unsigned long long *iter_acc = nullptr, *unconv_acc = nullptr;
   size_t lds_elem = 0, lds_cg = 0;
   LaunchConfig lc;
   // captured step graphs (element part, CG part) and the inputs they bake in
   bool use_graphs = true;
-  hipGraphExec_t g_elem = nullptr, g_cg = nullptr;
+  hipGraphExec_t g_elem = nullptr, g_fin = nullptr;
+  std::vector<hipGraphExec_t> g_chunks;  // CG iterations [j C, (j+1) C), built on first use
+  hipStream_t cap_stream = nullptr;      // stream used only for graph capture
+  int *done_host = nullptr;              // [2] pinned copies of the stopped-body count
+  hipEvent_t chunk_ev[2] = {nullptr, nullptr};
   std::vector<unsigned char> graph_key;
   std::vector<BodyState> last_body;
   bool pending = false;  // asynchronous steps enqueued since the last sync
@@ -112,6 +116,18 @@ void dfree(T *&p) {
   p = nullptr;
 }
 
+constexpr int kCgChunk = 32;  // CG iterations per captured graph
+
+void drop_graphs(otfem_solver *h) {
+  if (h->g_elem) (void)hipGraphExecDestroy(h->g_elem);
+  if (h->g_fin) (void)hipGraphExecDestroy(h->g_fin);
+  for (hipGraphExec_t g : h->g_chunks)
+    if (g) (void)hipGraphExecDestroy(g);
+  h->g_elem = h->g_fin = nullptr;
+  h->g_chunks.clear();
+  h->graph_key.clear();
+}
+
 void free_device(otfem_solver *h) {
   dfree(h->x); dfree(h->x0); dfree(h->v); dfree(h->fext); dfree(h->mass);
   dfree(h->rhs); dfree(h->r); dfree(h->p); dfree(h->y); dfree(h->fixed);
@@ -119,13 +135,18 @@ void free_device(otfem_solver *h) {
   dfree(h->rest64); dfree(h->vd64); dfree(h->bt4); dfree(h->bt8);
   dfree(h->Re); dfree(h->plastic); dfree(h->plastic_param);
   dfree(h->body_tile_begin); dfree(h->body); dfree(h->part_bb); dfree(h->part_rr);
-  dfree(h->part4); dfree(h->done_count); dfree(h->iter_acc);
+  dfree(h->part4); dfree(h->done_count); dfree(h->iter_acc); dfree(h->unconv_acc);
   dfree(h->vw); dfree(h->hist1); dfree(h->hist2); dfree(h->kd); dfree(h->free_rank);
   dfree(h->ws_bad);
-  if (h->g_elem) (void)hipGraphExecDestroy(h->g_elem);
-  if (h->g_cg) (void)hipGraphExecDestroy(h->g_cg);
-  h->g_elem = h->g_cg = nullptr;
-  h->graph_key.clear();
+  drop_graphs(h);
+  if (h->done_host) (void)hipHostFree(h->done_host);
+  h->done_host = nullptr;
+  for (hipEvent_t &e : h->chunk_ev) {
+    if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
+  if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
+  h->cap_stream = nullptr;
   for (hipEvent_t e : h->pool) (void)hipEventDestroy(e);
   h->pool.clear();
   h->pool_used = 0;
@@ -207,6 +228,10 @@ otfem_status ensure_device(otfem_solver *h) {
   const int64_t N = L.N, S = (int64_t)L.slot_tet.size(), T = L.T, B = L.B;
   const int64_t nt = (int64_t)L.tiles.size();
   HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  HIPCHK(h, hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
+  HIPCHK(h, hipHostMalloc(reinterpret_cast<void **>(&h->done_host), 2 * sizeof(int),
+                          hipHostMallocDefault));
+  for (hipEvent_t &e : h->chunk_ev) HIPCHK(h, hipEventCreateWithFlags(&e, hipEventDisableTiming));
   HIPCHK(h, hipEventCreate(&h->ev0));
   HIPCHK(h, hipEventCreate(&h->ev1));
   HIPCHK(h, hipEventCreate(&h->ev2));
@@ -237,6 +262,7 @@ otfem_status ensure_device(otfem_solver *h) {
   HIPCHK(h, dalloc(&h->part4, 8 * nt));
   HIPCHK(h, dalloc(&h->done_count, 1));
   HIPCHK(h, dalloc(&h->iter_acc, B));
+  HIPCHK(h, dalloc(&h->unconv_acc, B));
   hipStream_t s = h->stream;
   HIPCHK(h, hipMemcpyAsync(h->tiles, L.tiles.data(), nt * sizeof(TileInfo), hipMemcpyHostToDevice, s));
   if (!L.halo.empty())
@@ -270,6 +296,7 @@ otfem_status ensure_device(otfem_solver *h) {
   HIPCHK(h, hipMemsetAsync(h->part4, 0, 8 * nt * sizeof(double), s));
   HIPCHK(h, hipMemsetAsync(h->done_count, 0, sizeof(int), s));
   HIPCHK(h, hipMemsetAsync(h->iter_acc, 0, B * sizeof(unsigned long long), s));
+  HIPCHK(h, hipMemsetAsync(h->unconv_acc, 0, B * sizeof(unsigned long long), s));
   HIPCHK(h, hipStreamSynchronize(s));
   h->device_ready = true;
   return OTFEM_OK;
@@ -413,10 +440,7 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
   if (h->lds_elem > (size_t)prop.sharedMemPerBlock || h->lds_cg > (size_t)prop.sharedMemPerBlock)
     return fail(h, OTFEM_INVALID_ARG, "tile too large for LDS; lower target_tile_nodes");
   // material data changed: captured graphs are rebuilt on the next step
-  if (h->g_elem) (void)hipGraphExecDestroy(h->g_elem);
-  if (h->g_cg) (void)hipGraphExecDestroy(h->g_cg);
-  h->g_elem = h->g_cg = nullptr;
-  h->graph_key.clear();
+  drop_graphs(h);
   h->initialized = true;
   h->rotations_valid = false;
   st = ws_upload_kd(h);
@@ -433,7 +457,7 @@ StepArgs make_args(otfem_solver *h, double dt, int warp, const otfem_params &P) 
   a.Re = h->Re; a.plastic = h->plastic; a.plastic_param = h->plastic_param;
   a.body_tile_begin = h->body_tile_begin; a.body = h->body;
   a.part_bb = h->part_bb; a.part_rr = h->part_rr; a.part4 = h->part4;
-  a.done_count = h->done_count; a.iter_acc = h->iter_acc;
+  a.done_count = h->done_count; a.iter_acc = h->iter_acc; a.unconv_acc = h->unconv_acc;
   a.N = h->L.N;
   a.S = (int64_t)h->L.slot_tet.size();
   a.T = h->L.T;
@@ -801,41 +825,76 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
     HIPCHK(h, hipEventRecord(e[0], s));
   }
   if (ws_on) HIPCHK(h, enqueue_ws_predict(w, s));
+  const int cap = std::max(0, P.cg_max_iterations);
+  const int nchunks = (cap + kCgChunk - 1) / kCgChunk;
+  // capture `enqueue` (on the capture-only stream) into an executable graph
+  auto capture = [&](auto &&enqueue, hipGraphExec_t *out, const char *what) -> otfem_status {
+    hipGraph_t g;
+    HIPCHK(h, hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
+    hipError_t ce = enqueue(h->cap_stream);
+    hipError_t ee = hipStreamEndCapture(h->cap_stream, &g);
+    if (ce != hipSuccess) return hip_fail(h, ce, what);
+    HIPCHK(h, ee);
+    hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(h, ie);
+    return OTFEM_OK;
+  };
   if (h->use_graphs) {
     // the graphs bake in every kernel argument: rebuild when any changes
     std::vector<unsigned char> key(sizeof(StepArgs) + sizeof(LaunchConfig));
     std::memcpy(key.data(), &a, sizeof(StepArgs));
     std::memcpy(key.data() + sizeof(StepArgs), &h->lc, sizeof(LaunchConfig));
-    if (key != h->graph_key || !h->g_elem || !h->g_cg) {
-      if (h->g_elem) (void)hipGraphExecDestroy(h->g_elem);
-      if (h->g_cg) (void)hipGraphExecDestroy(h->g_cg);
-      h->g_elem = h->g_cg = nullptr;
-      h->graph_key.clear();
-      hipGraph_t g;
-      HIPCHK(h, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      hipError_t ce = enqueue_element(a, h->lds_elem, s);
-      hipError_t ee = hipStreamEndCapture(s, &g);
-      if (ce != hipSuccess) return hip_fail(h, ce, "enqueue_element");
-      HIPCHK(h, ee);
-      HIPCHK(h, hipGraphInstantiate(&h->g_elem, g, nullptr, nullptr, 0));
-      (void)hipGraphDestroy(g);
-      HIPCHK(h, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      ce = enqueue_cg(a, h->lc, h->lds_cg, s);
-      ee = hipStreamEndCapture(s, &g);
-      if (ce != hipSuccess) return hip_fail(h, ce, "enqueue_cg");
-      HIPCHK(h, ee);
-      HIPCHK(h, hipGraphInstantiate(&h->g_cg, g, nullptr, nullptr, 0));
-      (void)hipGraphDestroy(g);
+    if (key != h->graph_key || !h->g_elem || !h->g_fin) {
+      drop_graphs(h);
+      otfem_status cs = capture([&](hipStream_t cs_) { return enqueue_element(a, h->lds_elem, cs_); },
+                                &h->g_elem, "enqueue_element");
+      if (cs == OTFEM_OK)
+        cs = capture([&](hipStream_t cs_) { return enqueue_finish(a, h->lc, cs_); }, &h->g_fin,
+                     "enqueue_finish");
+      if (cs != OTFEM_OK) {
+        drop_graphs(h);
+        return cs;
+      }
+      h->g_chunks.assign(nchunks, nullptr);
       h->graph_key = key;
     }
     HIPCHK(h, hipGraphLaunch(h->g_elem, s));
-    if (timed) HIPCHK(h, hipEventRecord(e[1], s));
-    HIPCHK(h, hipGraphLaunch(h->g_cg, s));
   } else {
     HIPCHK(h, enqueue_element(a, h->lds_elem, s));
-    if (timed) HIPCHK(h, hipEventRecord(e[1], s));
-    HIPCHK(h, enqueue_cg(a, h->lc, h->lds_cg, s));
   }
+  if (timed) HIPCHK(h, hipEventRecord(e[1], s));
+  // CG in chunks of kCgChunk iterations.  The host keeps one chunk queued
+  // ahead of the one whose stopped-body count it reads back, and issues no
+  // more once every body has stopped; kernels of a chunk that runs after the
+  // stop exit at once.
+  for (int j = 0; j < nchunks; ++j) {
+    const int it0 = j * kCgChunk, it1 = std::min(cap, it0 + kCgChunk);
+    if (h->use_graphs) {
+      if (!h->g_chunks[j]) {
+        otfem_status cs = capture(
+            [&](hipStream_t cs_) { return enqueue_cg(a, h->lc, h->lds_cg, cs_, it0, it1); },
+            &h->g_chunks[j], "enqueue_cg");
+        if (cs != OTFEM_OK) return cs;
+      }
+      HIPCHK(h, hipGraphLaunch(h->g_chunks[j], s));
+    } else {
+      HIPCHK(h, enqueue_cg(a, h->lc, h->lds_cg, s, it0, it1));
+    }
+    if (j + 1 < nchunks) {
+      HIPCHK(h, hipMemcpyAsync(&h->done_host[j & 1], h->done_count, sizeof(int),
+                               hipMemcpyDeviceToHost, s));
+      HIPCHK(h, hipEventRecord(h->chunk_ev[j & 1], s));
+      if (j >= 1) {
+        HIPCHK(h, hipEventSynchronize(h->chunk_ev[(j - 1) & 1]));
+        if (*(volatile int *)&h->done_host[(j - 1) & 1] >= (int)h->L.B) break;
+      }
+    }
+  }
+  if (h->use_graphs)
+    HIPCHK(h, hipGraphLaunch(h->g_fin, s));
+  else
+    HIPCHK(h, enqueue_finish(a, h->lc, s));
   if (ws_on && h->recording) {
     HIPCHK(h, enqueue_ws_target(w, s));
     model_commit(h->model, h->n_free);
@@ -886,6 +945,16 @@ otfem_status otfem_iteration_totals(otfem_handle h, int64_t *totals, int reset) 
   if (totals)
     HIPCHK(h, hipMemcpy(totals, h->iter_acc, h->L.B * sizeof(int64_t), hipMemcpyDeviceToHost));
   if (reset) HIPCHK(h, hipMemset(h->iter_acc, 0, h->L.B * sizeof(unsigned long long)));
+  return OTFEM_OK;
+}
+
+otfem_status otfem_unconverged_totals(otfem_handle h, int64_t *totals, int reset) {
+  if (!h) return OTFEM_INVALID_ARG;
+  if (!h->device_ready) return fail(h, OTFEM_STATE, "no device state yet");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (totals)
+    HIPCHK(h, hipMemcpy(totals, h->unconv_acc, h->L.B * sizeof(int64_t), hipMemcpyDeviceToHost));
+  if (reset) HIPCHK(h, hipMemset(h->unconv_acc, 0, h->L.B * sizeof(unsigned long long)));
   return OTFEM_OK;
 }
 

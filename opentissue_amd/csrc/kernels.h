@@ -58,6 +58,7 @@ struct StepArgs {
   double *part4;         // [2][ntiles][4]  p.q, r.q, q.q, r.r per CG kernel
   int *done_count;       // bodies whose CG stopped in this step
   unsigned long long *iter_acc;  // [B] CG iterations summed over steps
+  unsigned long long *unconv_acc;  // [B] steps whose CG stopped unconverged
   // sizes
   int64_t N;
   int64_t S;
@@ -85,10 +86,14 @@ struct LaunchConfig {
   int spmv_mode = SPMV_F64;
 };
 
-// One step = enqueue_element, then enqueue_cg (cg_max x {S, U} kernels and
-// the finishing kernel).
+// One step = enqueue_element, then enqueue_cg for iterations [0, cg_max)
+// (one kernel per iteration; kernels of bodies that already stopped exit at
+// once, so the iteration range may be issued in chunks and cut short once
+// every body has stopped), then enqueue_finish.
 hipError_t enqueue_element(const StepArgs &a, size_t lds_elem, hipStream_t s);
-hipError_t enqueue_cg(const StepArgs &a, const LaunchConfig &lc, size_t lds_cg, hipStream_t s);
+hipError_t enqueue_cg(const StepArgs &a, const LaunchConfig &lc, size_t lds_cg, hipStream_t s,
+                      int it0, int it1);
+hipError_t enqueue_finish(const StepArgs &a, const LaunchConfig &lc, hipStream_t s);
 size_t element_lds_bytes(int max_nodes, int max_owned, int plastic_on);
 size_t cg_lds_bytes(int max_nodes, int max_owned);
 

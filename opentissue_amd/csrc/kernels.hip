@@ -787,8 +787,10 @@ __global__ __launch_bounds__(BLOCK) void k_finish(StepArgs A) {
     bs->rr = bs->rr0;
     bs->stopped = 1;
   }
-  if (ti.body_rank == 0 && threadIdx.x == 0)
+  if (ti.body_rank == 0 && threadIdx.x == 0) {
     A.iter_acc[ti.body] += (unsigned long long)(*(volatile int32_t *)&bs->iterations);
+    if (!*(volatile int32_t *)&bs->converged) A.unconv_acc[ti.body] += 1ull;
+  }
   for (int l = threadIdx.x; l < ti.n_owned; l += BLOCK) {
     const int64_t g = ti.node_begin + l;
     if (A.fixed[g]) continue;
@@ -810,21 +812,23 @@ size_t cg_lds_bytes(int max_nodes, int max_owned) {
 namespace {
 
 template <int BLOCK, typename MT, typename AT, typename ST>
-hipError_t enqueue_iterations(const StepArgs &a, size_t lds_cg, hipStream_t s) {
-  for (int it = 0; it < a.cg_max; ++it)
+hipError_t enqueue_iterations(const StepArgs &a, size_t lds_cg, hipStream_t s, int it0, int it1) {
+  for (int it = it0; it < it1; ++it)
     hipLaunchKernelGGL((k_cg_iter<BLOCK, MT, AT, ST>), dim3(a.ntiles), dim3(BLOCK), lds_cg, s, a,
                        it);
-  hipLaunchKernelGGL((k_finish<BLOCK>), dim3(a.ntiles), dim3(BLOCK), 0, s, a);
   return hipGetLastError();
 }
 
 template <int BLOCK>
-hipError_t enqueue_mode(const StepArgs &a, int mode, size_t lds_cg, hipStream_t s) {
+hipError_t enqueue_mode(const StepArgs &a, int mode, size_t lds_cg, hipStream_t s, int it0,
+                        int it1) {
   switch (mode) {
-    case SPMV_F32: return enqueue_iterations<BLOCK, float, float, float>(a, lds_cg, s);
-    case SPMV_F32_ACC64: return enqueue_iterations<BLOCK, float, double, float>(a, lds_cg, s);
-    case SPMV_F64: return enqueue_iterations<BLOCK, double, double, float>(a, lds_cg, s);
-    case SPMV_F64_STORE64: return enqueue_iterations<BLOCK, double, double, double>(a, lds_cg, s);
+    case SPMV_F32: return enqueue_iterations<BLOCK, float, float, float>(a, lds_cg, s, it0, it1);
+    case SPMV_F32_ACC64:
+      return enqueue_iterations<BLOCK, float, double, float>(a, lds_cg, s, it0, it1);
+    case SPMV_F64: return enqueue_iterations<BLOCK, double, double, float>(a, lds_cg, s, it0, it1);
+    case SPMV_F64_STORE64:
+      return enqueue_iterations<BLOCK, double, double, double>(a, lds_cg, s, it0, it1);
     default: return hipErrorInvalidValue;
   }
 }
@@ -836,13 +840,24 @@ hipError_t enqueue_element(const StepArgs &a, size_t lds_elem, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t enqueue_cg(const StepArgs &a, const LaunchConfig &lc, size_t lds_cg, hipStream_t s) {
+hipError_t enqueue_cg(const StepArgs &a, const LaunchConfig &lc, size_t lds_cg, hipStream_t s,
+                      int it0, int it1) {
   switch (lc.cg_block) {
-    case 256: return enqueue_mode<256>(a, lc.spmv_mode, lds_cg, s);
-    case 512: return enqueue_mode<512>(a, lc.spmv_mode, lds_cg, s);
-    case 1024: return enqueue_mode<1024>(a, lc.spmv_mode, lds_cg, s);
+    case 256: return enqueue_mode<256>(a, lc.spmv_mode, lds_cg, s, it0, it1);
+    case 512: return enqueue_mode<512>(a, lc.spmv_mode, lds_cg, s, it0, it1);
+    case 1024: return enqueue_mode<1024>(a, lc.spmv_mode, lds_cg, s, it0, it1);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t enqueue_finish(const StepArgs &a, const LaunchConfig &lc, hipStream_t s) {
+  switch (lc.cg_block) {
+    case 256: hipLaunchKernelGGL((k_finish<256>), dim3(a.ntiles), dim3(256), 0, s, a); break;
+    case 512: hipLaunchKernelGGL((k_finish<512>), dim3(a.ntiles), dim3(512), 0, s, a); break;
+    case 1024: hipLaunchKernelGGL((k_finish<1024>), dim3(a.ntiles), dim3(1024), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 }  // namespace otfem

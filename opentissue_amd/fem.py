@@ -113,6 +113,7 @@ SIGNATURES = {
     "otfem_set_timing": (C.c_int, [_H, C.c_int]),
     "otfem_get_timing": (C.c_int, [_H, _dp, _dp, C.POINTER(C.c_longlong)]),
     "otfem_iteration_totals": (C.c_int, [_H, C.POINTER(C.c_int64), C.c_int]),
+    "otfem_unconverged_totals": (C.c_int, [_H, C.POINTER(C.c_int64), C.c_int]),
     "otfem_set_kernel_config": (C.c_int, [_H, C.c_int, C.c_int, C.c_int]),
     "otfem_autotune": (C.c_int, [_H, C.c_double, C.c_int, C.POINTER(Params), C.c_int,
                                  C.POINTER(C.c_int)]),
@@ -326,6 +327,12 @@ class Solver:
         tot = np.empty(self.B, dtype=np.int64)
         self._check(self.lib.otfem_iteration_totals(self.h, tot.ctypes.data_as(C.POINTER(C.c_int64)),
                                                     1 if reset else 0))
+        return tot
+
+    def unconverged_totals(self, reset=False):
+        tot = np.empty(self.B, dtype=np.int64)
+        self._check(self.lib.otfem_unconverged_totals(self.h, tot.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                      1 if reset else 0))
         return tot
 
     def set_kernel_config(self, cg_block=256, spmv_mode=1, use_graphs=True):
