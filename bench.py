@@ -318,7 +318,11 @@ def main():
         wl = time.perf_counter() - t0
         itl = s.iteration_totals()
         unl = int(s.unconverged_totals().sum())
-        el_ms, cgl_ms, nl = s.timing()
+        el_ms, cgl_ms, ws_ms, nl = s.timing4()
+        # network roofline: nominal 2 * 34,688 MAC per node per forward pass
+        net_flops = 2.0 * 34688 * N
+        net_ms = ws_ms / max(1, nl)
+        net_tflops = net_flops / (net_ms / 1e3) / 1e12 if net_ms > 0 else 0.0
         s.set_warmstart_model(None)
         vl = T * args.steps / wl
         log(f"learned start: {vl:.4g} tet-steps/s ({vl / value:.3f}x plain), "
@@ -330,8 +334,16 @@ def main():
             "vs_plain": vl / value,
             "cg_iterations_per_step": float(itl.sum()) / args.steps / w.num_bodies,
             "cg_body_steps_unconverged": unl,
-            "kernel_ms_per_step": {"element_and_network": el_ms / max(1, nl),
+            "kernel_ms_per_step": {"network": net_ms, "element": el_ms / max(1, nl),
                                    "cg": cgl_ms / max(1, nl)},
+            "network_roofline": {"bound": "mfma", "kernel": "k_ws_predict",
+                                 "achieved": net_tflops, "peak": 2500.0, "unit": "TFLOP/s",
+                                 "frac": net_tflops / 2500.0,
+                                 "algorithmic_flops_per_launch": net_flops,
+                                 "avg_launch_ms": net_ms,
+                                 "note": "nominal BF16 flops; the kernel issues 2.2x as many "
+                                         "(inputs and activations as BF16 hi + lo pairs, "
+                                         "3-output layer padded to 32 rows)"},
             "flags": int(stl.flags),
             "training": info,
             "ab_uncapped": ab,

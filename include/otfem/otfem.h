@@ -140,12 +140,12 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
  * and any error of the enqueued steps are reported by otfem_synchronize.  No
  * reference counterpart: fem::simulate is synchronous. */
 otfem_status otfem_synchronize(otfem_handle h, otfem_stats *stats);
-/* Per-kernel HIP-event timing of every step (element kernel, CG kernel),
- * accumulated until the next otfem_set_timing call.  No reference
- * counterpart (fem has no timing hooks). */
+/* Per-kernel HIP-event timing of every step (element kernel, CG kernels,
+ * and the learned-start network when it runs), accumulated until the next
+ * otfem_set_timing call.  No reference counterpart (fem has no timing hooks). */
 otfem_status otfem_set_timing(otfem_handle h, int enable);
 otfem_status otfem_get_timing(otfem_handle h, double *element_ms, double *cg_ms,
-                              long long *steps);
+                              double *network_ms, long long *steps);
 /* CG iterations per body summed over all steps since the last reset
  * (reset != 0 clears after reading; totals may be NULL).  No reference
  * counterpart; used to count algorithmic bytes of asynchronous steps. */

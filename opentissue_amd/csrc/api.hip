@@ -69,7 +69,7 @@ struct otfem_solver {
   std::vector<BodyState> last_body;
   bool pending = false;  // asynchronous steps enqueued since the last sync
   double last_elem_ms = 0.0, last_cg_ms = 0.0;
-  double acc_elem_ms = 0.0, acc_cg_ms = 0.0;
+  double acc_elem_ms = 0.0, acc_cg_ms = 0.0, acc_ws_ms = 0.0;
   long long timed_steps = 0;
   bool timing = false;
   // per-step event triples of asynchronous timed steps, resolved at sync
@@ -480,10 +480,14 @@ StepArgs make_args(otfem_solver *h, double dt, int warp, const otfem_params &P) 
 }
 
 otfem_status resolve_timing(otfem_solver *h) {
-  for (size_t k = 0; 3 * k + 2 < h->pool_used; ++k) {
-    float e1 = 0.f, e2 = 0.f;
-    HIPCHK(h, hipEventElapsedTime(&e1, h->pool[3 * k], h->pool[3 * k + 1]));
-    HIPCHK(h, hipEventElapsedTime(&e2, h->pool[3 * k + 1], h->pool[3 * k + 2]));
+  // four events per timed step: start, after the network, after the
+  // element kernel, after the CG kernels
+  for (size_t k = 0; 4 * k + 3 < h->pool_used; ++k) {
+    float e0 = 0.f, e1 = 0.f, e2 = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&e0, h->pool[4 * k], h->pool[4 * k + 1]));
+    HIPCHK(h, hipEventElapsedTime(&e1, h->pool[4 * k + 1], h->pool[4 * k + 2]));
+    HIPCHK(h, hipEventElapsedTime(&e2, h->pool[4 * k + 2], h->pool[4 * k + 3]));
+    h->acc_ws_ms += e0;
     h->last_elem_ms = e1;
     h->last_cg_ms = e2;
     h->acc_elem_ms += e1;
@@ -813,18 +817,19 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
   if (learned) a.vw = h->vw;
   hipStream_t s = h->stream;
   const bool timed = (stats != nullptr) || h->timing;
-  hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
   if (timed) {
-    while (h->pool.size() < h->pool_used + 3) {
+    while (h->pool.size() < h->pool_used + 4) {
       hipEvent_t ev;
       HIPCHK(h, hipEventCreate(&ev));
       h->pool.push_back(ev);
     }
-    for (int k = 0; k < 3; ++k) e[k] = h->pool[h->pool_used + k];
-    h->pool_used += 3;
+    for (int k = 0; k < 4; ++k) e[k] = h->pool[h->pool_used + k];
+    h->pool_used += 4;
     HIPCHK(h, hipEventRecord(e[0], s));
   }
   if (ws_on) HIPCHK(h, enqueue_ws_predict(w, s));
+  if (timed) HIPCHK(h, hipEventRecord(e[1], s));
   const int cap = std::max(0, P.cg_max_iterations);
   const int nchunks = (cap + kCgChunk - 1) / kCgChunk;
   // capture `enqueue` (on the capture-only stream) into an executable graph
@@ -863,7 +868,7 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
   } else {
     HIPCHK(h, enqueue_element(a, h->lds_elem, s));
   }
-  if (timed) HIPCHK(h, hipEventRecord(e[1], s));
+  if (timed) HIPCHK(h, hipEventRecord(e[2], s));
   // CG in chunks of kCgChunk iterations.  The host keeps one chunk queued
   // ahead of the one whose stopped-body count it reads back, and issues no
   // more once every body has stopped; kernels of a chunk that runs after the
@@ -899,7 +904,7 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
     HIPCHK(h, enqueue_ws_target(w, s));
     model_commit(h->model, h->n_free);
   }
-  if (timed) HIPCHK(h, hipEventRecord(e[2], s));
+  if (timed) HIPCHK(h, hipEventRecord(e[3], s));
   h->rotations_valid = (P.store_rotations != 0) && use_stiffness_warping;
   h->pending = true;
   if (!stats) return OTFEM_OK;
@@ -919,13 +924,13 @@ otfem_status otfem_set_timing(otfem_handle h, int enable) {
     h->pool_used = 0;
   }
   h->timing = enable != 0;
-  h->acc_elem_ms = h->acc_cg_ms = 0.0;
+  h->acc_elem_ms = h->acc_cg_ms = h->acc_ws_ms = 0.0;
   h->timed_steps = 0;
   return OTFEM_OK;
 }
 
 otfem_status otfem_get_timing(otfem_handle h, double *element_ms, double *cg_ms,
-                              long long *steps) {
+                              double *network_ms, long long *steps) {
   if (!h) return OTFEM_INVALID_ARG;
   if (h->device_ready && h->pool_used) {
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -933,6 +938,7 @@ otfem_status otfem_get_timing(otfem_handle h, double *element_ms, double *cg_ms,
     if (ts != OTFEM_OK) return ts;
   }
   if (element_ms) *element_ms = h->acc_elem_ms;
+  if (network_ms) *network_ms = h->acc_ws_ms;
   if (cg_ms) *cg_ms = h->acc_cg_ms;
   if (steps) *steps = h->timed_steps;
   return OTFEM_OK;

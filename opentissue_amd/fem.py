@@ -111,7 +111,7 @@ SIGNATURES = {
     "otfem_step": (C.c_int, [_H, C.c_double, C.c_int, C.POINTER(Params), C.POINTER(Stats)]),
     "otfem_synchronize": (C.c_int, [_H, C.POINTER(Stats)]),
     "otfem_set_timing": (C.c_int, [_H, C.c_int]),
-    "otfem_get_timing": (C.c_int, [_H, _dp, _dp, C.POINTER(C.c_longlong)]),
+    "otfem_get_timing": (C.c_int, [_H, _dp, _dp, _dp, C.POINTER(C.c_longlong)]),
     "otfem_iteration_totals": (C.c_int, [_H, C.POINTER(C.c_int64), C.c_int]),
     "otfem_unconverged_totals": (C.c_int, [_H, C.POINTER(C.c_int64), C.c_int]),
     "otfem_set_kernel_config": (C.c_int, [_H, C.c_int, C.c_int, C.c_int]),
@@ -317,11 +317,18 @@ class Solver:
         self._check(self.lib.otfem_set_timing(self.h, 1 if enable else 0))
 
     def timing(self):
+        """(element ms, CG ms, steps) accumulated since set_timing."""
+        e, c, w, n = self.timing4()
+        return e, c, n
+
+    def timing4(self):
+        """(element ms, CG ms, learned-start network ms, steps)."""
         e = C.c_double()
         c = C.c_double()
+        w = C.c_double()
         n = C.c_longlong()
-        self._check(self.lib.otfem_get_timing(self.h, C.byref(e), C.byref(c), C.byref(n)))
-        return e.value, c.value, n.value
+        self._check(self.lib.otfem_get_timing(self.h, C.byref(e), C.byref(c), C.byref(w), C.byref(n)))
+        return e.value, c.value, w.value, n.value
 
     def iteration_totals(self, reset=False):
         tot = np.empty(self.B, dtype=np.int64)
