@@ -268,8 +268,10 @@ otfem_status ensure_device(otfem_solver *h) {
   if (!L.halo.empty())
     HIPCHK(h, hipMemcpyAsync(h->halo, L.halo.data(), L.halo.size() * sizeof(int32_t),
                              hipMemcpyHostToDevice, s));
-  HIPCHK(h, hipMemcpyAsync(h->conn, L.slot_conn.data(), S * sizeof(ushort4), hipMemcpyHostToDevice, s));
-  HIPCHK(h, hipMemcpyAsync(h->slot_tet, L.slot_tet.data(), S * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  if (S > 0) {  // a mesh may have no tets (every node fixed)
+    HIPCHK(h, hipMemcpyAsync(h->conn, L.slot_conn.data(), S * sizeof(ushort4), hipMemcpyHostToDevice, s));
+    HIPCHK(h, hipMemcpyAsync(h->slot_tet, L.slot_tet.data(), S * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  }
   HIPCHK(h, hipMemcpyAsync(h->body_tile_begin, L.body_tile_begin.data(), (B + 1) * sizeof(int32_t),
                            hipMemcpyHostToDevice, s));
   std::vector<double> tmp;
@@ -386,6 +388,12 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
     double amount = h->density[t] * ed[t].V * .25;
     for (int k = 0; k < 4; ++k) h->mass_host[h->tets[4 * t + k]] += amount;
   }
+  // A free node outside every tetrahedron has zero mass and an empty row of A;
+  // the reference would divide by zero in CG.  Reject it (fix or drop it).
+  for (int64_t i = 0; i < N; ++i)
+    if (!h->fixed_host[i] && !(h->mass_host[i] > 0.0))
+      return fail(h, OTFEM_INVALID_ARG,
+                  "free node " + std::to_string(i) + " belongs to no tetrahedron (zero mass)");
   // Diagonal of the rest stiffness K_ii (R = I): per tet and corner k with
   // gradient b, (Ke_kk)_dd = D0 b_d^2 + D2 (|b|^2 - b_d^2) (fem_compute_ke.h:78-99,
   // with D = V (d16, d17, d18)); feature of the learned warm start.
@@ -426,10 +434,13 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
   }
   h->plastic_on = (c_creep > 0.0 && c_max > 0.0);
   hipStream_t s = h->stream;
-  HIPCHK(h, hipMemcpyAsync(h->rest64, rest.data(), 9 * S * sizeof(double), hipMemcpyHostToDevice, s));
-  HIPCHK(h, hipMemcpyAsync(h->vd64, vd.data(), 3 * S * sizeof(double), hipMemcpyHostToDevice, s));
+  if (S > 0) {
+    HIPCHK(h, hipMemcpyAsync(h->rest64, rest.data(), 9 * S * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(h, hipMemcpyAsync(h->vd64, vd.data(), 3 * S * sizeof(double), hipMemcpyHostToDevice, s));
+  }
   HIPCHK(h, hipMemcpyAsync(h->mass, mint.data(), N * sizeof(double), hipMemcpyHostToDevice, s));
-  HIPCHK(h, hipMemcpyAsync(h->plastic_param, pp.data(), 3 * T * sizeof(double), hipMemcpyHostToDevice, s));
+  if (T > 0)
+    HIPCHK(h, hipMemcpyAsync(h->plastic_param, pp.data(), 3 * T * sizeof(double), hipMemcpyHostToDevice, s));
   HIPCHK(h, hipMemsetAsync(h->plastic, 0, 6 * T * sizeof(double), s));
   HIPCHK(h, hipStreamSynchronize(s));
 
