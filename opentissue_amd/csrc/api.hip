@@ -54,16 +54,18 @@ struct otfem_solver {
   int32_t *body_tile_begin = nullptr;
   BodyState *body = nullptr;
   double *part_bb = nullptr, *part_rr = nullptr, *part4 = nullptr;
-  int *done_count = nullptr;
+  int *done_count = nullptr;  // [2]: stopped bodies, active tiles
+  int32_t *act = nullptr;     // [ntiles] active-tile list of batched meshes
   unsigned long long *iter_acc = nullptr, *unconv_acc = nullptr;
   size_t lds_elem = 0, lds_cg = 0;
   LaunchConfig lc;
   // captured step graphs (element part, CG part) and the inputs they bake in
   bool use_graphs = true;
   hipGraphExec_t g_elem = nullptr, g_fin = nullptr;
-  std::vector<hipGraphExec_t> g_chunks;  // CG iterations [j C, (j+1) C), built on first use
+  std::map<std::pair<int, int>, hipGraphExec_t> g_chunks;  // (chunk, grid) -> CG iterations
+                                                          // [j C, (j+1) C), built on first use
   hipStream_t cap_stream = nullptr;      // stream used only for graph capture
-  int *done_host = nullptr;              // [2] pinned copies of the stopped-body count
+  int *done_host = nullptr;              // [2][2] pinned copies of done_count
   hipEvent_t chunk_ev[2] = {nullptr, nullptr};
   std::vector<unsigned char> graph_key;
   std::vector<BodyState> last_body;
@@ -121,8 +123,8 @@ constexpr int kCgChunk = 32;  // CG iterations per captured graph
 void drop_graphs(otfem_solver *h) {
   if (h->g_elem) (void)hipGraphExecDestroy(h->g_elem);
   if (h->g_fin) (void)hipGraphExecDestroy(h->g_fin);
-  for (hipGraphExec_t g : h->g_chunks)
-    if (g) (void)hipGraphExecDestroy(g);
+  for (auto &kv : h->g_chunks)
+    if (kv.second) (void)hipGraphExecDestroy(kv.second);
   h->g_elem = h->g_fin = nullptr;
   h->g_chunks.clear();
   h->graph_key.clear();
@@ -229,7 +231,7 @@ otfem_status ensure_device(otfem_solver *h) {
   const int64_t nt = (int64_t)L.tiles.size();
   HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   HIPCHK(h, hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
-  HIPCHK(h, hipHostMalloc(reinterpret_cast<void **>(&h->done_host), 2 * sizeof(int),
+  HIPCHK(h, hipHostMalloc(reinterpret_cast<void **>(&h->done_host), 4 * sizeof(int),
                           hipHostMallocDefault));
   for (hipEvent_t &e : h->chunk_ev) HIPCHK(h, hipEventCreateWithFlags(&e, hipEventDisableTiming));
   HIPCHK(h, hipEventCreate(&h->ev0));
@@ -260,7 +262,8 @@ otfem_status ensure_device(otfem_solver *h) {
   HIPCHK(h, dalloc(&h->part_bb, nt));
   HIPCHK(h, dalloc(&h->part_rr, nt));
   HIPCHK(h, dalloc(&h->part4, 8 * nt));
-  HIPCHK(h, dalloc(&h->done_count, 1));
+  HIPCHK(h, dalloc(&h->done_count, 2));
+  HIPCHK(h, dalloc(&h->act, nt));
   HIPCHK(h, dalloc(&h->iter_acc, B));
   HIPCHK(h, dalloc(&h->unconv_acc, B));
   hipStream_t s = h->stream;
@@ -296,7 +299,7 @@ otfem_status ensure_device(otfem_solver *h) {
   HIPCHK(h, hipMemsetAsync(h->part_bb, 0, nt * sizeof(double), s));
   HIPCHK(h, hipMemsetAsync(h->part_rr, 0, nt * sizeof(double), s));
   HIPCHK(h, hipMemsetAsync(h->part4, 0, 8 * nt * sizeof(double), s));
-  HIPCHK(h, hipMemsetAsync(h->done_count, 0, sizeof(int), s));
+  HIPCHK(h, hipMemsetAsync(h->done_count, 0, 2 * sizeof(int), s));
   HIPCHK(h, hipMemsetAsync(h->iter_acc, 0, B * sizeof(unsigned long long), s));
   HIPCHK(h, hipMemsetAsync(h->unconv_acc, 0, B * sizeof(unsigned long long), s));
   HIPCHK(h, hipStreamSynchronize(s));
@@ -469,6 +472,8 @@ StepArgs make_args(otfem_solver *h, double dt, int warp, const otfem_params &P) 
   a.body_tile_begin = h->body_tile_begin; a.body = h->body;
   a.part_bb = h->part_bb; a.part_rr = h->part_rr; a.part4 = h->part4;
   a.done_count = h->done_count; a.iter_acc = h->iter_acc; a.unconv_acc = h->unconv_acc;
+  a.act = h->act;
+  a.compact = h->L.B > 1 ? 1 : 0;
   a.N = h->L.N;
   a.S = (int64_t)h->L.slot_tet.size();
   a.T = h->L.T;
@@ -872,7 +877,6 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
         drop_graphs(h);
         return cs;
       }
-      h->g_chunks.assign(nchunks, nullptr);
       h->graph_key = key;
     }
     HIPCHK(h, hipGraphLaunch(h->g_elem, s));
@@ -884,26 +888,42 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
   // ahead of the one whose stopped-body count it reads back, and issues no
   // more once every body has stopped; kernels of a chunk that runs after the
   // stop exit at once.
+  // Batched meshes also shrink the CG grid to the active tiles: the count
+  // read back (one chunk behind) only falls within a step, so a grid sized
+  // from it covers the list the next chunk builds; grids are powers of two
+  // times 256, capped at the tile count.
+  const int ntiles = (int)h->L.tiles.size();
+  int known_active = ntiles;
   for (int j = 0; j < nchunks; ++j) {
     const int it0 = j * kCgChunk, it1 = std::min(cap, it0 + kCgChunk);
+    int grid = ntiles;
+    if (a.compact) {
+      grid = 256;
+      while (grid < known_active) grid *= 2;
+      grid = std::min(grid, ntiles);
+    }
     if (h->use_graphs) {
-      if (!h->g_chunks[j]) {
+      hipGraphExec_t &ge = h->g_chunks[std::make_pair(j, grid)];
+      if (!ge) {
         otfem_status cs = capture(
-            [&](hipStream_t cs_) { return enqueue_cg(a, h->lc, h->lds_cg, cs_, it0, it1); },
-            &h->g_chunks[j], "enqueue_cg");
+            [&](hipStream_t cs_) { return enqueue_cg(a, h->lc, h->lds_cg, cs_, it0, it1, grid); },
+            &ge, "enqueue_cg");
         if (cs != OTFEM_OK) return cs;
       }
-      HIPCHK(h, hipGraphLaunch(h->g_chunks[j], s));
+      HIPCHK(h, hipGraphLaunch(ge, s));
     } else {
-      HIPCHK(h, enqueue_cg(a, h->lc, h->lds_cg, s, it0, it1));
+      HIPCHK(h, enqueue_cg(a, h->lc, h->lds_cg, s, it0, it1, grid));
     }
     if (j + 1 < nchunks) {
-      HIPCHK(h, hipMemcpyAsync(&h->done_host[j & 1], h->done_count, sizeof(int),
+      HIPCHK(h, hipMemcpyAsync(&h->done_host[2 * (j & 1)], h->done_count, 2 * sizeof(int),
                                hipMemcpyDeviceToHost, s));
       HIPCHK(h, hipEventRecord(h->chunk_ev[j & 1], s));
       if (j >= 1) {
         HIPCHK(h, hipEventSynchronize(h->chunk_ev[(j - 1) & 1]));
-        if (*(volatile int *)&h->done_host[(j - 1) & 1] >= (int)h->L.B) break;
+        const volatile int *dh = &h->done_host[2 * ((j - 1) & 1)];
+        if (dh[0] >= (int)h->L.B) break;
+        const int act_now = dh[1];
+        if (a.compact) known_active = std::min(known_active, std::max(act_now, 1));
       }
     }
   }

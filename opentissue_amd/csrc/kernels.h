@@ -56,7 +56,9 @@ struct StepArgs {
   double *part_bb;       // [ntiles]  b.b (element kernel)
   double *part_rr;       // [ntiles]  r_0.r_0 (element kernel)
   double *part4;         // [2][ntiles][4]  p.q, r.q, q.q, r.r per CG kernel
-  int *done_count;       // bodies whose CG stopped in this step
+  int *done_count;       // [2]: bodies whose CG stopped in this step, active tiles
+  int32_t *act;          // [ntiles] tiles of running bodies (k_compact)
+  int32_t compact;       // CG workgroups map onto act[] (batched meshes)
   unsigned long long *iter_acc;  // [B] CG iterations summed over steps
   unsigned long long *unconv_acc;  // [B] steps whose CG stopped unconverged
   // sizes
@@ -91,8 +93,10 @@ struct LaunchConfig {
 // once, so the iteration range may be issued in chunks and cut short once
 // every body has stopped), then enqueue_finish.
 hipError_t enqueue_element(const StepArgs &a, size_t lds_elem, hipStream_t s);
+// With a.compact, the range starts with k_compact and the CG kernels run
+// `grid` workgroups over the active-tile list (grid >= active tiles).
 hipError_t enqueue_cg(const StepArgs &a, const LaunchConfig &lc, size_t lds_cg, hipStream_t s,
-                      int it0, int it1);
+                      int it0, int it1, int grid);
 hipError_t enqueue_finish(const StepArgs &a, const LaunchConfig &lc, hipStream_t s);
 size_t element_lds_bytes(int max_nodes, int max_owned, int plastic_on);
 size_t cg_lds_bytes(int max_nodes, int max_owned);

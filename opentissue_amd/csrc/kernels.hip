@@ -610,7 +610,15 @@ __global__ __launch_bounds__(BLOCK) void k_cg_iter(StepArgs A, int it) {
   extern __shared__ double lds[];
   __shared__ double red[4 * (BLOCK / 64 + 1)];
   if (*(volatile int *)A.done_count >= A.B) return;
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  int tile;
+  if (A.compact) {
+    // workgroup -> active tile; the remap keeps neighbouring tiles on one XCD
+    const int nact = *(volatile int *)(A.done_count + 1);
+    if ((int)blockIdx.x >= nact) return;
+    tile = A.act[xcd_remap(blockIdx.x, nact)];
+  } else {
+    tile = xcd_remap(blockIdx.x, gridDim.x);
+  }
   const TileInfo ti = A.tiles[tile];
   BodyState *bs = &A.body[ti.body];
   if (*(volatile int32_t *)&bs->stopped) return;
@@ -812,23 +820,51 @@ size_t cg_lds_bytes(int max_nodes, int max_owned) {
 namespace {
 
 template <int BLOCK, typename MT, typename AT, typename ST>
-hipError_t enqueue_iterations(const StepArgs &a, size_t lds_cg, hipStream_t s, int it0, int it1) {
+hipError_t enqueue_iterations(const StepArgs &a, size_t lds_cg, hipStream_t s, int it0, int it1,
+                              int grid) {
   for (int it = it0; it < it1; ++it)
-    hipLaunchKernelGGL((k_cg_iter<BLOCK, MT, AT, ST>), dim3(a.ntiles), dim3(BLOCK), lds_cg, s, a,
-                       it);
+    hipLaunchKernelGGL((k_cg_iter<BLOCK, MT, AT, ST>), dim3(grid), dim3(BLOCK), lds_cg, s, a, it);
   return hipGetLastError();
+}
+
+// Active-tile list: tiles whose body has not stopped, in tile order, and
+// their count in done_count[1].  One workgroup; each thread scans a
+// contiguous range of tiles.
+constexpr int kCompactBlock = 1024;
+__global__ __launch_bounds__(kCompactBlock) void k_compact(StepArgs A) {
+  __shared__ int scan[kCompactBlock];
+  const int nt = A.ntiles;
+  const int per = (nt + kCompactBlock - 1) / kCompactBlock;
+  const int t0 = threadIdx.x * per, t1 = min(nt, t0 + per);
+  int cnt = 0;
+  for (int t = t0; t < t1; ++t)
+    cnt += *(volatile int32_t *)&A.body[A.tiles[t].body].stopped ? 0 : 1;
+  scan[threadIdx.x] = cnt;
+  __syncthreads();
+  // inclusive Hillis-Steele scan
+  for (int off = 1; off < kCompactBlock; off <<= 1) {
+    const int v = threadIdx.x >= off ? scan[threadIdx.x - off] : 0;
+    __syncthreads();
+    scan[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int pos = scan[threadIdx.x] - cnt;
+  for (int t = t0; t < t1; ++t)
+    if (!*(volatile int32_t *)&A.body[A.tiles[t].body].stopped) A.act[pos++] = t;
+  if (threadIdx.x == kCompactBlock - 1) A.done_count[1] = scan[kCompactBlock - 1];
 }
 
 template <int BLOCK>
 hipError_t enqueue_mode(const StepArgs &a, int mode, size_t lds_cg, hipStream_t s, int it0,
-                        int it1) {
+                        int it1, int g) {
   switch (mode) {
-    case SPMV_F32: return enqueue_iterations<BLOCK, float, float, float>(a, lds_cg, s, it0, it1);
+    case SPMV_F32: return enqueue_iterations<BLOCK, float, float, float>(a, lds_cg, s, it0, it1, g);
     case SPMV_F32_ACC64:
-      return enqueue_iterations<BLOCK, float, double, float>(a, lds_cg, s, it0, it1);
-    case SPMV_F64: return enqueue_iterations<BLOCK, double, double, float>(a, lds_cg, s, it0, it1);
+      return enqueue_iterations<BLOCK, float, double, float>(a, lds_cg, s, it0, it1, g);
+    case SPMV_F64:
+      return enqueue_iterations<BLOCK, double, double, float>(a, lds_cg, s, it0, it1, g);
     case SPMV_F64_STORE64:
-      return enqueue_iterations<BLOCK, double, double, double>(a, lds_cg, s, it0, it1);
+      return enqueue_iterations<BLOCK, double, double, double>(a, lds_cg, s, it0, it1, g);
     default: return hipErrorInvalidValue;
   }
 }
@@ -841,11 +877,16 @@ hipError_t enqueue_element(const StepArgs &a, size_t lds_elem, hipStream_t s) {
 }
 
 hipError_t enqueue_cg(const StepArgs &a, const LaunchConfig &lc, size_t lds_cg, hipStream_t s,
-                      int it0, int it1) {
+                      int it0, int it1, int grid) {
+  if (a.compact) {
+    hipLaunchKernelGGL(k_compact, dim3(1), dim3(kCompactBlock), 0, s, a);
+  } else {
+    grid = a.ntiles;
+  }
   switch (lc.cg_block) {
-    case 256: return enqueue_mode<256>(a, lc.spmv_mode, lds_cg, s, it0, it1);
-    case 512: return enqueue_mode<512>(a, lc.spmv_mode, lds_cg, s, it0, it1);
-    case 1024: return enqueue_mode<1024>(a, lc.spmv_mode, lds_cg, s, it0, it1);
+    case 256: return enqueue_mode<256>(a, lc.spmv_mode, lds_cg, s, it0, it1, grid);
+    case 512: return enqueue_mode<512>(a, lc.spmv_mode, lds_cg, s, it0, it1, grid);
+    case 1024: return enqueue_mode<1024>(a, lc.spmv_mode, lds_cg, s, it0, it1, grid);
     default: return hipErrorInvalidValue;
   }
 }
