@@ -47,6 +47,7 @@ namespace otfem {
 namespace {
 
 constexpr int kElemBlock = 256;
+constexpr int kSlotsInFlight = 4;  // FP32 slot records loaded per thread before use in k_cg_iter
 
 __device__ __forceinline__ int xcd_remap(int bid, int nb) {
   // Workgroups are dispatched round-robin over the 8 XCDs.  Give each XCD a
@@ -554,6 +555,49 @@ __device__ __forceinline__ void load_slot<double, double>(const StepArgs &A, int
   }
 }
 
+// Raw slot record as loaded (conversion to the math type happens at use, so
+// several records can be in flight per thread at little register cost).
+template <typename ST>
+struct SlotRaw;
+template <>
+struct SlotRaw<float> {
+  ushort4 c;
+  float4 q[3];
+};
+template <>
+struct SlotRaw<double> {
+  ushort4 c;
+  double4 q[3];
+};
+
+template <typename ST>
+__device__ __forceinline__ void load_raw(const StepArgs &A, int64_t S, int s, SlotRaw<ST> &r);
+template <>
+__device__ __forceinline__ void load_raw<float>(const StepArgs &A, int64_t S, int s,
+                                                SlotRaw<float> &r) {
+  r.c = A.conn[s];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) r.q[k] = A.bt4[k * S + s];
+}
+template <>
+__device__ __forceinline__ void load_raw<double>(const StepArgs &A, int64_t S, int s,
+                                                 SlotRaw<double> &r) {
+  r.c = A.conn[s];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) r.q[k] = A.bt8[k * S + s];
+}
+
+template <typename MT, typename ST>
+__device__ __forceinline__ void unpack_raw(const SlotRaw<ST> &r, MT bt[3][3], MT D[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    bt[k][0] = (MT)r.q[k].x;
+    bt[k][1] = (MT)r.q[k].y;
+    bt[k][2] = (MT)r.q[k].z;
+    D[k] = (MT)r.q[k].w;
+  }
+}
+
 // y_owned += (K' p)_e for one slot: G = sum_k (p_k - p_0) bt_k^T,
 // sigma = D(G), f_k = sigma bt_k, f_0 = -(f_1 + f_2 + f_3).
 template <typename MT, typename AT>
@@ -707,24 +751,22 @@ __global__ __launch_bounds__(BLOCK) void k_cg_iter(StepArgs A, int it) {
   if (stop) return;
   __syncthreads();
 
-  // q = K' p over the tile's slots, two slots per thread in flight
+  // q = K' p over the tile's slots, kSlotsInFlight raw records per thread
+  // loaded before any is used
+  constexpr int U = sizeof(ST) == 8 ? 2 : kSlotsInFlight;  // FP64 records: fewer in flight
   const int64_t S = A.S;
   const int send = ti.slot_begin + ti.n_slots;
-  for (int s = ti.slot_begin + threadIdx.x; s < send; s += 2 * BLOCK) {
-    const int s2 = s + BLOCK;
-    const bool two = s2 < send;
-    ushort4 c[2];
-    MT bt[2][3][3], D[2][3];
-    c[0] = A.conn[s];
-    load_slot<MT, ST>(A, S, s, bt[0], D[0]);
-    if (two) {
-      c[1] = A.conn[s2];
-      load_slot<MT, ST>(A, S, s2, bt[1], D[1]);
-    }
+  for (int s = ti.slot_begin + threadIdx.x; s < send; s += U * BLOCK) {
+    SlotRaw<ST> raw[U];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      if (u == 1 && !two) break;
-      slot_spmv<MT, AT>(c[u], bt[u], D[u], sp, sy, no);
+    for (int u = 0; u < U; ++u)
+      if (s + u * BLOCK < send) load_raw<ST>(A, S, s + u * BLOCK, raw[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (s + u * BLOCK >= send) break;
+      MT bt[3][3], D[3];
+      unpack_raw<MT, ST>(raw[u], bt, D);
+      slot_spmv<MT, AT>(raw[u].c, bt, D, sp, sy, no);
     }
   }
   __syncthreads();

