@@ -16,7 +16,7 @@ struct BodyState {
   double rr;      // r.r at the last recorded point
   double d;       // r.r entering the current iteration (for alpha)
   int32_t iterations;
-  int32_t stopped;
+  int32_t stopped;  // 0 while running; else 1 + index of the CG kernel that recorded the stop
   int32_t converged;
   int32_t flags;  // bit 0: non-finite scalar
 };

@@ -508,13 +508,15 @@ __device__ __forceinline__ bool stop_rule(const StepArgs &A, int it, double rr, 
   return converged || it >= A.cg_max;
 }
 
+// `kernel` is the CG kernel that records the stop; `stopped` keeps kernel + 1
+// so that workgroups of that same kernel still run their prologue.
 __device__ __forceinline__ void record_stop(const StepArgs &A, BodyState *w, int it, double rr,
-                                            bool converged, bool nonfinite) {
+                                            bool converged, bool nonfinite, int kernel) {
   w->iterations = it;
   w->rr = rr;
   w->converged = converged ? 1 : 0;
   w->flags |= nonfinite ? 1 : 0;
-  w->stopped = 1;
+  w->stopped = kernel + 1;
   __threadfence();
   atomicAdd(A.done_count, 1);
 }
@@ -640,6 +642,122 @@ __device__ __forceinline__ void slot_spmv(ushort4 c, const MT bt[3][3], const MT
 
 }  // namespace
 
+// Per-thread register budget of k_cg_iter, by block size: tile nodes whose
+// loads are all issued before any is used, and slot records per buffer of
+// the double-buffered slot loop (FP64 records take twice the registers).
+template <int BLOCK>
+struct CgShape {
+  static constexpr int kNodes = BLOCK <= 256 ? 4 : (BLOCK <= 512 ? 2 : 1);
+  static constexpr int kSlots = BLOCK <= 256 ? kSlotsInFlight : (BLOCK <= 512 ? 2 : 1);
+};
+
+// One tile node's inputs to the CG prologue, as loaded.
+struct NodeIn {
+  int32_t g;  // internal node id
+  int32_t fx;
+  double m;
+  double r[3], p[3], y[3], v[3];
+};
+
+// Loads the inputs of tile nodes l = threadIdx.x + k*BLOCK (k < NB) without
+// using any of them, so every load of the batch is in flight at once.  Lanes
+// past the node table read node_begin (in bounds) and are never used.
+template <int BLOCK, int NB>
+__device__ __forceinline__ void load_nodes(const StepArgs &A, const TileInfo &ti, int it,
+                                           const double *Rp, const double *Pp,
+                                           const double *Yp, NodeIn (&nd)[NB]) {
+  const int nn = ti.n_owned + ti.n_halo, no = ti.n_owned;
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    const int l = (int)threadIdx.x + k * BLOCK;
+    int g = ti.node_begin;
+    if (l < no)
+      g = ti.node_begin + l;
+    else if (l < nn)
+      g = __ldg(&A.halo[ti.halo_begin + l - no]);
+    nd[k].g = g;
+  }
+  const double *R = it == 0 ? A.r : Rp;
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    const int64_t g = nd[k].g;
+    const int l = (int)threadIdx.x + k * BLOCK;
+    nd[k].fx = A.fixed[g];
+    nd[k].m = A.mass[g];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) nd[k].r[d] = R[3 * g + d];
+    if (it > 0) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        nd[k].p[d] = Pp[3 * g + d];
+        nd[k].y[d] = Yp[3 * g + d];
+        nd[k].v[d] = l < no ? A.v[3 * g + d] : 0.0;
+      }
+    }
+  }
+}
+
+// Completes iteration it-1 for one tile node and forms the new direction:
+// r = r - alpha q, p = r + beta p (q = cM m p + cK y), v += alpha p for owned
+// nodes; p goes to LDS for the slot loop, r and p of owned nodes to LDS and
+// memory.  Fixed nodes carry zero r and p.
+template <typename AT>
+__device__ __forceinline__ void update_node(const StepArgs &A, int it, bool stop, double alpha,
+                                            double beta, int l, int no, const NodeIn &n,
+                                            double *Rn, double *Pn, double *sp, double *sr,
+                                            AT *sy) {
+  const int64_t g = n.g;
+  double rn[3], pn[3];
+  if (n.fx) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) rn[d] = pn[d] = 0.0;
+  } else if (it == 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) rn[d] = pn[d] = n.r[d];
+  } else {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double q = A.cM * n.m * n.p[d] + A.cK * n.y[d];
+      rn[d] = n.r[d] - alpha * q;
+      pn[d] = rn[d] + beta * n.p[d];
+    }
+    if (l < no) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) A.v[3 * g + d] = n.v[d] + alpha * n.p[d];
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) sp[3 * l + d] = pn[d];
+  if (l < no) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (it > 0 && !stop) Rn[3 * g + d] = rn[d];
+      Pn[3 * g + d] = pn[d];
+      sr[3 * l + d] = rn[d];
+      sy[3 * l + d] = (AT)0;
+    }
+  }
+}
+
+// Owned node l's contribution to the tile partials p.q, r.q, q.q, r.r, and
+// its K'p to memory.
+template <typename AT>
+__device__ __forceinline__ void node_partials(const StepArgs &A, int64_t g, int l, double m,
+                                              const double *sp, const double *sr, const AT *sy,
+                                              double *Yn, double4 &acc) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double yd = (double)sy[3 * l + d];
+    Yn[3 * g + d] = yd;
+    const double p = sp[3 * l + d], r = sr[3 * l + d];
+    const double q = A.cM * m * p + A.cK * yd;
+    acc.x += p * q;
+    acc.y += r * q;
+    acc.z += q * q;
+    acc.w += r * r;
+  }
+}
+
 // One CG iteration per kernel (fem_conjugate_gradients.h:69-132).  Kernel j
 // first completes iteration j-1 -- alpha = r.r / p.Ap from the previous
 // kernel's partials (:101-105), v += alpha p (:113), r -= alpha q (:114) --
@@ -649,11 +767,18 @@ __device__ __forceinline__ void slot_spmv(ushort4 c, const MT bt[3][3], const MT
 // r'.r' = r.r - 2 alpha r.q + alpha^2 q.q, so the stop test and beta need no
 // extra pass; alpha itself always uses the directly summed r.r.
 // Buffers alternate with j: kernel j reads set (j-1)&1 and writes set j&1.
+//
+// Latency: the loads that do not depend on alpha and beta -- the first slot
+// records and the node inputs of up to kNodes nodes per thread -- are issued
+// before the body's partials are summed, so the kernel waits for memory about
+// three times (tile record, node ids, everything else) rather than once per
+// node loop trip.  The slot loop is double-buffered.
 template <int BLOCK, typename MT, typename AT, typename ST>
 __global__ __launch_bounds__(BLOCK) void k_cg_iter(StepArgs A, int it) {
   extern __shared__ double lds[];
   __shared__ double red[4 * (BLOCK / 64 + 1)];
-  if (*(volatile int *)A.done_count >= A.B) return;
+  constexpr int NB = CgShape<BLOCK>::kNodes;
+  constexpr int U = sizeof(ST) == 8 ? (CgShape<BLOCK>::kSlots + 1) / 2 : CgShape<BLOCK>::kSlots;
   int tile;
   if (A.compact) {
     // workgroup -> active tile; the remap keeps neighbouring tiles on one XCD
@@ -665,16 +790,36 @@ __global__ __launch_bounds__(BLOCK) void k_cg_iter(StepArgs A, int it) {
   }
   const TileInfo ti = A.tiles[tile];
   BodyState *bs = &A.body[ti.body];
-  if (*(volatile int32_t *)&bs->stopped) return;
+  // exit if the body stopped in an earlier kernel (a stop recorded by this
+  // kernel still needs this workgroup's prologue: v += alpha p)
+  const int st = *(volatile int32_t *)&bs->stopped;
+  if (st != 0 && st - 1 < it) return;
   const int t0 = A.body_tile_begin[ti.body], t1 = A.body_tile_begin[ti.body + 1];
   const int nt = A.ntiles;
-  const double *prevp = A.part4 + ((it - 1) & 1) * 4 * (int64_t)nt;
+  const int nn = ti.n_owned + ti.n_halo, no = ti.n_owned;
+  const int cur = it & 1, prv = (it - 1) & 1;
+  double *Rn = A.r + cur * 3 * A.N, *Pn = A.p + cur * 3 * A.N, *Yn = A.y + cur * 3 * A.N;
+  const double *Rp = A.r + prv * 3 * A.N, *Pp = A.p + prv * 3 * A.N, *Yp = A.y + prv * 3 * A.N;
+
+  // first slot records; loads past the tile's last slot re-read that slot
+  const int64_t S = A.S;
+  const int sb = ti.slot_begin, send = sb + ti.n_slots;
+  const bool has_slots = ti.n_slots > 0;
+  SlotRaw<ST> ra[U], rb[U];
+  if (has_slots) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      load_raw<ST>(A, S, min(sb + (int)threadIdx.x + u * BLOCK, send - 1), ra[u]);
+  }
+  NodeIn nd[NB];
+  load_nodes<BLOCK, NB>(A, ti, it, Rp, Pp, Yp, nd);
 
   double alpha = 0.0, beta = 0.0, rr, bb;
   if (it == 0) {
     rr = body_sum<BLOCK>(A.part_rr, t0, t1, red);
     bb = body_sum<BLOCK>(A.part_bb, t0, t1, red);
   } else {
+    const double *prevp = A.part4 + prv * 4 * (int64_t)nt;
     const double4 s4 = body_sum4<BLOCK>(reinterpret_cast<const double4 *>(prevp), t0, t1, red);
     double pq = s4.x;
     const double rq = s4.y, qq = s4.z, rrd = s4.w;
@@ -682,7 +827,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_iter(StepArgs A, int it) {
     if (fabs(pq) < A.tiny) pq = A.tiny;  // fem_conjugate_gradients.h:101-102
     alpha = rrd / pq;
     if (!finite(alpha)) {
-      if (ti.body_rank == 0 && threadIdx.x == 0) record_stop(A, bs, it - 1, rrd, false, true);
+      if (ti.body_rank == 0 && threadIdx.x == 0) record_stop(A, bs, it - 1, rrd, false, true, it);
       return;
     }
     rr = fmax(rrd - 2.0 * alpha * rq + alpha * alpha * qq, 0.0);
@@ -698,97 +843,82 @@ __global__ __launch_bounds__(BLOCK) void k_cg_iter(StepArgs A, int it) {
       bs->rr0 = rr;
     }
     if (stop)
-      record_stop(A, bs, it, rr, conv, nonfin);
+      record_stop(A, bs, it, rr, conv, nonfin, it);
     else
       bs->iterations = it;
   }
 
-  const int nn = ti.n_owned + ti.n_halo, no = ti.n_owned;
-  const int cur = it & 1, prv = (it - 1) & 1;
-  double *Rn = A.r + cur * 3 * A.N, *Pn = A.p + cur * 3 * A.N, *Yn = A.y + cur * 3 * A.N;
-  const double *Rp = A.r + prv * 3 * A.N, *Pp = A.p + prv * 3 * A.N, *Yp = A.y + prv * 3 * A.N;
   double *sp = lds;                 // p of every tile node
   double *sr = sp + 3 * nn;         // r of owned nodes
   AT *sy = reinterpret_cast<AT *>(sr + 3 * no);  // K'p of owned nodes
 
   // prologue: finish iteration it-1 and form the new direction
-  for (int l = threadIdx.x; l < nn; l += BLOCK) {
-    const int64_t g = l < no ? ti.node_begin + l : __ldg(&A.halo[ti.halo_begin + l - no]);
-    const bool fx = A.fixed[g] != 0;
-    double rn[3], pn[3];
-    if (it == 0) {
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        rn[d] = fx ? 0.0 : A.r[3 * g + d];
-        pn[d] = rn[d];
-      }
-    } else if (fx) {
+  for (int k = 0; k < NB; ++k) {
+    const int l = (int)threadIdx.x + k * BLOCK;
+    if (l < nn) update_node<AT>(A, it, stop, alpha, beta, l, no, nd[k], Rn, Pn, sp, sr, sy);
+  }
+  for (int l = (int)threadIdx.x + NB * BLOCK; l < nn; l += BLOCK) {
+    // tiles larger than NB nodes per thread
+    NodeIn n;
+    n.g = l < no ? ti.node_begin + l : __ldg(&A.halo[ti.halo_begin + l - no]);
+    const int64_t g = n.g;
+    n.fx = A.fixed[g];
+    n.m = A.mass[g];
+    const double *R = it == 0 ? A.r : Rp;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) rn[d] = pn[d] = 0.0;
-    } else {
-      const double m = A.mass[g];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const double pk = Pp[3 * g + d];
-        const double q = A.cM * m * pk + A.cK * Yp[3 * g + d];
-        rn[d] = Rp[3 * g + d] - alpha * q;
-        pn[d] = rn[d] + beta * pk;
-        if (l < no) A.v[3 * g + d] += alpha * pk;
-      }
+    for (int d = 0; d < 3; ++d) {
+      n.r[d] = R[3 * g + d];
+      n.p[d] = it > 0 ? Pp[3 * g + d] : 0.0;
+      n.y[d] = it > 0 ? Yp[3 * g + d] : 0.0;
+      n.v[d] = (it > 0 && l < no) ? A.v[3 * g + d] : 0.0;
     }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) sp[3 * l + d] = pn[d];
-    if (l < no) {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        if (it > 0 && !stop) Rn[3 * g + d] = rn[d];
-        Pn[3 * g + d] = pn[d];
-        sr[3 * l + d] = rn[d];
-        sy[3 * l + d] = (AT)0;
-      }
-    }
+    update_node<AT>(A, it, stop, alpha, beta, l, no, n, Rn, Pn, sp, sr, sy);
   }
   if (stop) return;
   __syncthreads();
 
-  // q = K' p over the tile's slots, kSlotsInFlight raw records per thread
-  // loaded before any is used
-  constexpr int U = sizeof(ST) == 8 ? 2 : kSlotsInFlight;  // FP64 records: fewer in flight
-  const int64_t S = A.S;
-  const int send = ti.slot_begin + ti.n_slots;
-  for (int s = ti.slot_begin + threadIdx.x; s < send; s += U * BLOCK) {
-    SlotRaw<ST> raw[U];
+  // q = K' p over the tile's slots: buffer ra is in use while rb loads, and
+  // the other way round
+  if (has_slots) {
+    for (int s = sb + (int)threadIdx.x; s < send; s += 2 * U * BLOCK) {
 #pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (s + u * BLOCK < send) load_raw<ST>(A, S, s + u * BLOCK, raw[u]);
+      for (int u = 0; u < U; ++u) load_raw<ST>(A, S, min(s + (U + u) * BLOCK, send - 1), rb[u]);
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (s + u * BLOCK >= send) break;
-      MT bt[3][3], D[3];
-      unpack_raw<MT, ST>(raw[u], bt, D);
-      slot_spmv<MT, AT>(raw[u].c, bt, D, sp, sy, no);
+      for (int u = 0; u < U; ++u) {
+        if (s + u * BLOCK < send) {
+          MT bt[3][3], D[3];
+          unpack_raw<MT, ST>(ra[u], bt, D);
+          slot_spmv<MT, AT>(ra[u].c, bt, D, sp, sy, no);
+        }
+      }
+      if (s + U * BLOCK >= send) break;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        load_raw<ST>(A, S, min(s + (2 * U + u) * BLOCK, send - 1), ra[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (s + (U + u) * BLOCK < send) {
+          MT bt[3][3], D[3];
+          unpack_raw<MT, ST>(rb[u], bt, D);
+          slot_spmv<MT, AT>(rb[u].c, bt, D, sp, sy, no);
+        }
+      }
     }
   }
   __syncthreads();
 
-  double pq = 0.0, rq = 0.0, qq = 0.0, rrd = 0.0;
-  for (int l = threadIdx.x; l < no; l += BLOCK) {
-    const int64_t g = ti.node_begin + l;
-    if (A.fixed[g]) continue;
-    const double m = A.mass[g];
+  double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const double yd = (double)sy[3 * l + d];
-      Yn[3 * g + d] = yd;
-      const double p = sp[3 * l + d], r = sr[3 * l + d];
-      const double q = A.cM * m * p + A.cK * yd;
-      pq += p * q;
-      rq += r * q;
-      qq += q * q;
-      rrd += r * r;
-    }
+  for (int k = 0; k < NB; ++k) {
+    const int l = (int)threadIdx.x + k * BLOCK;
+    if (l < no && !nd[k].fx) node_partials<AT>(A, nd[k].g, l, nd[k].m, sp, sr, sy, Yn, acc);
   }
-  const double4 tot = block_sum4<BLOCK>(make_double4(pq, rq, qq, rrd), red);
+  for (int l = (int)threadIdx.x + NB * BLOCK; l < no; l += BLOCK) {
+    const int64_t g = ti.node_begin + l;
+    if (!A.fixed[g]) node_partials<AT>(A, g, l, A.mass[g], sp, sr, sy, Yn, acc);
+  }
+  const double4 tot = block_sum4<BLOCK>(acc, red);
   if (threadIdx.x == 0) reinterpret_cast<double4 *>(A.part4 + cur * 4 * (int64_t)nt)[tile] = tot;
 }
 
