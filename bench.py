@@ -192,6 +192,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm start legs")
     ap.add_argument("--cap", type=int, default=5000, help="CG iteration cap (safety net)")
+    ap.add_argument("--cg-block", type=int, default=0, choices=(0, 256, 512, 1024),
+                    help="threads per tile in the CG kernels (0: autotune)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -214,8 +216,13 @@ def main():
     params_kw = dict(alpha=0.1, beta=0.001, tol=1e-6, cap=args.cap)
     P = fem.config_params(**params_kw)
     layout = s.layout_info()
-    best_block = s.autotune(w.dt, True, P, steps=2)
-    log(f"autotuned CG block size: {best_block} threads per tile ({layout['tiles']} tiles)")
+    if args.cg_block:
+        best_block = args.cg_block
+        s.set_kernel_config(cg_block=best_block)
+        log(f"CG block size: {best_block} threads per tile ({layout['tiles']} tiles)")
+    else:
+        best_block = s.autotune(w.dt, True, P, steps=2)
+        log(f"autotuned CG block size: {best_block} threads per tile ({layout['tiles']} tiles)")
 
     # warmup, then back to rest so the timed steps are steps 1..K from rest
     for _ in range(args.warmup):

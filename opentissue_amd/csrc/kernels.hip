@@ -47,7 +47,6 @@ namespace otfem {
 namespace {
 
 constexpr int kElemBlock = 256;
-constexpr int kSlotsInFlight = 4;  // FP32 slot records loaded per thread before use in k_cg_iter
 
 __device__ __forceinline__ int xcd_remap(int bid, int nb) {
   // Workgroups are dispatched round-robin over the 8 XCDs.  Give each XCD a
@@ -642,13 +641,16 @@ __device__ __forceinline__ void slot_spmv(ushort4 c, const MT bt[3][3], const MT
 
 }  // namespace
 
-// Per-thread register budget of k_cg_iter, by block size: tile nodes whose
-// loads are all issued before any is used, and slot records per buffer of
-// the double-buffered slot loop (FP64 records take twice the registers).
+// Per-thread register budget of k_cg_iter, by block size: tile nodes per
+// thread whose loads are all issued before any is used (one batch), slot
+// records per buffer of the double-buffered slot loop (FP64 records take
+// twice the registers), and the waves per SIMD the register allocation must
+// allow.  512 threads at 4 waves per SIMD run two tiles per CU.
 template <int BLOCK>
 struct CgShape {
   static constexpr int kNodes = BLOCK <= 256 ? 4 : (BLOCK <= 512 ? 2 : 1);
-  static constexpr int kSlots = BLOCK <= 256 ? kSlotsInFlight : (BLOCK <= 512 ? 2 : 1);
+  static constexpr int kSlots = BLOCK <= 256 ? 2 : 1;
+  static constexpr int kWaves = BLOCK <= 256 ? 2 : 4;
 };
 
 // One tile node's inputs to the CG prologue, as loaded.
@@ -659,17 +661,18 @@ struct NodeIn {
   double r[3], p[3], y[3], v[3];
 };
 
-// Loads the inputs of tile nodes l = threadIdx.x + k*BLOCK (k < NB) without
-// using any of them, so every load of the batch is in flight at once.  Lanes
-// past the node table read node_begin (in bounds) and are never used.
+// Loads the inputs of tile nodes l = base + threadIdx.x + k*BLOCK (k < NB)
+// without using any of them, so every load of the batch is in flight at
+// once.  Lanes past the node table read node_begin (in bounds) and are never
+// used.
 template <int BLOCK, int NB>
 __device__ __forceinline__ void load_nodes(const StepArgs &A, const TileInfo &ti, int it,
-                                           const double *Rp, const double *Pp,
+                                           int base, const double *Rp, const double *Pp,
                                            const double *Yp, NodeIn (&nd)[NB]) {
   const int nn = ti.n_owned + ti.n_halo, no = ti.n_owned;
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
-    const int l = (int)threadIdx.x + k * BLOCK;
+    const int l = base + (int)threadIdx.x + k * BLOCK;
     int g = ti.node_begin;
     if (l < no)
       g = ti.node_begin + l;
@@ -681,7 +684,7 @@ __device__ __forceinline__ void load_nodes(const StepArgs &A, const TileInfo &ti
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
     const int64_t g = nd[k].g;
-    const int l = (int)threadIdx.x + k * BLOCK;
+    const int l = base + (int)threadIdx.x + k * BLOCK;
     nd[k].fx = A.fixed[g];
     nd[k].m = A.mass[g];
 #pragma unroll
@@ -700,12 +703,13 @@ __device__ __forceinline__ void load_nodes(const StepArgs &A, const TileInfo &ti
 // Completes iteration it-1 for one tile node and forms the new direction:
 // r = r - alpha q, p = r + beta p (q = cM m p + cK y), v += alpha p for owned
 // nodes; p goes to LDS for the slot loop, r and p of owned nodes to LDS and
-// memory.  Fixed nodes carry zero r and p.
+// memory, and the mass of owned nodes to LDS (-1 marks a fixed node).  Fixed
+// nodes carry zero r and p.
 template <typename AT>
 __device__ __forceinline__ void update_node(const StepArgs &A, int it, bool stop, double alpha,
                                             double beta, int l, int no, const NodeIn &n,
                                             double *Rn, double *Pn, double *sp, double *sr,
-                                            AT *sy) {
+                                            AT *sy, double *sm) {
   const int64_t g = n.g;
   double rn[3], pn[3];
   if (n.fx) {
@@ -736,6 +740,7 @@ __device__ __forceinline__ void update_node(const StepArgs &A, int it, bool stop
       sr[3 * l + d] = rn[d];
       sy[3 * l + d] = (AT)0;
     }
+    sm[l] = n.fx ? -1.0 : n.m;
   }
 }
 
@@ -774,7 +779,8 @@ __device__ __forceinline__ void node_partials(const StepArgs &A, int64_t g, int 
 // three times (tile record, node ids, everything else) rather than once per
 // node loop trip.  The slot loop is double-buffered.
 template <int BLOCK, typename MT, typename AT, typename ST>
-__global__ __launch_bounds__(BLOCK) void k_cg_iter(StepArgs A, int it) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(CgShape<BLOCK>::kWaves)))
+void k_cg_iter(StepArgs A, int it) {
   extern __shared__ double lds[];
   __shared__ double red[4 * (BLOCK / 64 + 1)];
   constexpr int NB = CgShape<BLOCK>::kNodes;
@@ -812,7 +818,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_iter(StepArgs A, int it) {
       load_raw<ST>(A, S, min(sb + (int)threadIdx.x + u * BLOCK, send - 1), ra[u]);
   }
   NodeIn nd[NB];
-  load_nodes<BLOCK, NB>(A, ti, it, Rp, Pp, Yp, nd);
+  load_nodes<BLOCK, NB>(A, ti, it, 0, Rp, Pp, Yp, nd);
 
   double alpha = 0.0, beta = 0.0, rr, bb;
   if (it == 0) {
@@ -850,30 +856,20 @@ __global__ __launch_bounds__(BLOCK) void k_cg_iter(StepArgs A, int it) {
 
   double *sp = lds;                 // p of every tile node
   double *sr = sp + 3 * nn;         // r of owned nodes
-  AT *sy = reinterpret_cast<AT *>(sr + 3 * no);  // K'p of owned nodes
+  double *sm = sr + 3 * no;         // mass of owned nodes, -1 if fixed
+  AT *sy = reinterpret_cast<AT *>(sm + no);  // K'p of owned nodes
 
-  // prologue: finish iteration it-1 and form the new direction
+  // prologue: finish iteration it-1 and form the new direction, NB nodes per
+  // thread at a time (the first batch was loaded above)
+  for (int base = 0;;) {
 #pragma unroll
-  for (int k = 0; k < NB; ++k) {
-    const int l = (int)threadIdx.x + k * BLOCK;
-    if (l < nn) update_node<AT>(A, it, stop, alpha, beta, l, no, nd[k], Rn, Pn, sp, sr, sy);
-  }
-  for (int l = (int)threadIdx.x + NB * BLOCK; l < nn; l += BLOCK) {
-    // tiles larger than NB nodes per thread
-    NodeIn n;
-    n.g = l < no ? ti.node_begin + l : __ldg(&A.halo[ti.halo_begin + l - no]);
-    const int64_t g = n.g;
-    n.fx = A.fixed[g];
-    n.m = A.mass[g];
-    const double *R = it == 0 ? A.r : Rp;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      n.r[d] = R[3 * g + d];
-      n.p[d] = it > 0 ? Pp[3 * g + d] : 0.0;
-      n.y[d] = it > 0 ? Yp[3 * g + d] : 0.0;
-      n.v[d] = (it > 0 && l < no) ? A.v[3 * g + d] : 0.0;
+    for (int k = 0; k < NB; ++k) {
+      const int l = base + (int)threadIdx.x + k * BLOCK;
+      if (l < nn) update_node<AT>(A, it, stop, alpha, beta, l, no, nd[k], Rn, Pn, sp, sr, sy, sm);
     }
-    update_node<AT>(A, it, stop, alpha, beta, l, no, n, Rn, Pn, sp, sr, sy);
+    base += NB * BLOCK;
+    if (base >= nn) break;
+    load_nodes<BLOCK, NB>(A, ti, it, base, Rp, Pp, Yp, nd);
   }
   if (stop) return;
   __syncthreads();
@@ -909,14 +905,9 @@ __global__ __launch_bounds__(BLOCK) void k_cg_iter(StepArgs A, int it) {
   __syncthreads();
 
   double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
-#pragma unroll
-  for (int k = 0; k < NB; ++k) {
-    const int l = (int)threadIdx.x + k * BLOCK;
-    if (l < no && !nd[k].fx) node_partials<AT>(A, nd[k].g, l, nd[k].m, sp, sr, sy, Yn, acc);
-  }
-  for (int l = (int)threadIdx.x + NB * BLOCK; l < no; l += BLOCK) {
-    const int64_t g = ti.node_begin + l;
-    if (!A.fixed[g]) node_partials<AT>(A, g, l, A.mass[g], sp, sr, sy, Yn, acc);
+  for (int l = threadIdx.x; l < no; l += BLOCK) {
+    const double m = sm[l];
+    if (m >= 0.0) node_partials<AT>(A, ti.node_begin + l, l, m, sp, sr, sy, Yn, acc);
   }
   const double4 tot = block_sum4<BLOCK>(acc, red);
   if (threadIdx.x == 0) reinterpret_cast<double4 *>(A.part4 + cur * 4 * (int64_t)nt)[tile] = tot;
@@ -984,9 +975,9 @@ size_t element_lds_bytes(int max_nodes, int max_owned, int plastic_on) {
 }
 
 size_t cg_lds_bytes(int max_nodes, int max_owned) {
-  // FP64 p of all tile nodes, FP64 r of owned nodes, K'p accumulators
-  // (FP64 covers every mode)
-  return sizeof(double) * 3 * (size_t)max_nodes + 2 * sizeof(double) * 3 * (size_t)max_owned;
+  // FP64 p of all tile nodes; FP64 r, mass and K'p accumulators of owned
+  // nodes (FP64 covers every mode)
+  return sizeof(double) * 3 * (size_t)max_nodes + sizeof(double) * 7 * (size_t)max_owned;
 }
 
 namespace {

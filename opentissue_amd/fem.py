@@ -88,6 +88,7 @@ _H = C.c_void_p
 _M = C.c_void_p
 
 WARM_PREVIOUS_VELOCITY, WARM_LEARNED = 0, 1
+SPMV_F32, SPMV_F32_ACC64, SPMV_F64, SPMV_F64_STORE64 = 0, 1, 2, 3
 MLP_INPUTS, MLP_OUTPUTS, MLP_PARAMS = 12, 3, 35075
 SIGNATURES = {
     "otfem_version": (C.c_char_p, []),
@@ -342,7 +343,9 @@ class Solver:
                                                       1 if reset else 0))
         return tot
 
-    def set_kernel_config(self, cg_block=256, spmv_mode=1, use_graphs=True):
+    def set_kernel_config(self, cg_block=256, spmv_mode=SPMV_F64, use_graphs=True):
+        """Threads per tile in the CG kernels (256, 512, 1024) and the SpMV
+        precision (SPMV_*; the library's default is SPMV_F64)."""
         self._check(self.lib.otfem_set_kernel_config(self.h, cg_block, spmv_mode, 1 if use_graphs else 0))
 
     def autotune(self, dt, warp=True, params=None, steps=2):

@@ -84,6 +84,42 @@ def test_config_mode_per_step(dims):
     assert errs.max() <= TOL, errs
 
 
+@pytest.mark.parametrize("block", [256, 512, 1024])
+@pytest.mark.parametrize("tile_nodes", [0, 1200])
+def test_cg_block_sizes_and_node_batches(block, tile_nodes):
+    """Every CG block size, on tiles whose node table fits one register batch
+    per thread (default tiling) and on tiles that need several (1,200 owned
+    nodes per tile: node tables above 1,024 nodes)."""
+    from opentissue_amd import fem
+    import oracle
+    c, t, fx = _lattice(10, 40, 10)
+    s, o = _pair(c, t, fx, target_tile_nodes=tile_nodes)
+    s.set_kernel_config(cg_block=block)
+    if tile_nodes:
+        assert s.layout_info()["max_tile_nodes"] > 1024
+    errs, iters = _per_step_errors(s, o, c, 4, 0.01, fem.config_params(), oracle.config_params())
+    print(block, tile_nodes, errs, iters)
+    assert errs.max() <= TOL, errs
+
+
+@pytest.mark.parametrize("mode,tol", [("SPMV_F64", TOL), ("SPMV_F64_STORE64", 1e-7)])
+def test_spmv_precision_modes(mode, tol):
+    """The default SpMV (FP64 arithmetic on FP32 element data) meets the 1e-6
+    bar.  With the element data in FP64 too, what is left is summation order:
+    the GPU's per-node sums are LDS atomics in wave order, the oracle's are
+    sequential, and 130-odd CG iterations carry that difference (measured on
+    this mesh: 9e-9 on the first step from rest, ~3e-12 on later steps, at
+    identical iteration counts)."""
+    from opentissue_amd import fem
+    import oracle
+    c, t, fx = _lattice(10, 40, 10)
+    s, o = _pair(c, t, fx)
+    s.set_kernel_config(cg_block=256, spmv_mode=getattr(fem, mode))
+    errs, iters = _per_step_errors(s, o, c, 4, 0.01, fem.config_params(), oracle.config_params())
+    print(mode, errs, iters)
+    assert errs.max() <= tol, errs
+
+
 def test_rotations_match_oracle():
     from opentissue_amd import fem
     import oracle
