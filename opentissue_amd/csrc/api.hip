@@ -50,6 +50,8 @@ struct otfem_solver {
   double *rest64 = nullptr, *vd64 = nullptr;
   float4 *bt4 = nullptr;
   double4 *bt8 = nullptr;
+  double4 *tile_d = nullptr;  // [ntiles] D per unit volume and unit E of a tile's one nu
+  bool one_nu_per_tile = false;  // slot records may use the folded form
   double *Re = nullptr, *plastic = nullptr, *plastic_param = nullptr;
   int32_t *body_tile_begin = nullptr;
   BodyState *body = nullptr;
@@ -134,7 +136,7 @@ void free_device(otfem_solver *h) {
   dfree(h->x); dfree(h->x0); dfree(h->v); dfree(h->fext); dfree(h->mass);
   dfree(h->rhs); dfree(h->r); dfree(h->p); dfree(h->y); dfree(h->fixed);
   dfree(h->tiles); dfree(h->halo); dfree(h->conn); dfree(h->slot_tet);
-  dfree(h->rest64); dfree(h->vd64); dfree(h->bt4); dfree(h->bt8);
+  dfree(h->rest64); dfree(h->vd64); dfree(h->bt4); dfree(h->bt8); dfree(h->tile_d);
   dfree(h->Re); dfree(h->plastic); dfree(h->plastic_param);
   dfree(h->body_tile_begin); dfree(h->body); dfree(h->part_bb); dfree(h->part_rr);
   dfree(h->part4); dfree(h->done_count); dfree(h->iter_acc); dfree(h->unconv_acc);
@@ -254,6 +256,7 @@ otfem_status ensure_device(otfem_solver *h) {
   HIPCHK(h, dalloc(&h->rest64, 9 * S));
   HIPCHK(h, dalloc(&h->vd64, 3 * S));
   HIPCHK(h, dalloc(&h->bt4, 3 * S));
+  HIPCHK(h, dalloc(&h->tile_d, nt));
   HIPCHK(h, dalloc(&h->Re, 9 * T));
   HIPCHK(h, dalloc(&h->plastic, 6 * T));
   HIPCHK(h, dalloc(&h->plastic_param, 3 * T));
@@ -316,6 +319,15 @@ struct ElemData {
   double V;  // signed volume (compute_volume edge form)
 };
 
+// Isotropic material constants of fem_compute_ke.h:74-77 (d16, d17, d18):
+// D per unit volume.
+void material_d(double E, double nu, double d[3]) {
+  const double d15 = E / (1.0 + nu) / (1.0 - 2 * nu);
+  d[0] = (1.0 - nu) * d15;
+  d[1] = nu * d15;
+  d[2] = E / 2 / (1.0 + nu);
+}
+
 ElemData element_data(const double *p0, const double *p1, const double *p2, const double *p3,
                       double E, double nu) {
   ElemData e;
@@ -335,13 +347,11 @@ ElemData element_data(const double *p0, const double *p1, const double *p2, cons
     e.grad[3 + c] = -B2[c];
     e.grad[6 + c] = -B3[c];
   }
-  double d15 = E / (1.0 + nu) / (1.0 - 2 * nu);
-  double d16 = (1.0 - nu) * d15;
-  double d17 = nu * d15;
-  double d18 = E / 2 / (1.0 + nu);
-  e.vd[0] = d16 * d14;
-  e.vd[1] = d17 * d14;
-  e.vd[2] = d18 * d14;
+  double dm[3];
+  material_d(E, nu, dm);
+  e.vd[0] = dm[0] * d14;
+  e.vd[1] = dm[1] * d14;
+  e.vd[2] = dm[2] * d14;
   // e10 . (e20 % e30) / 6 with operator% of math_vector3.h:193
   double cx = d6 * d11 - d7 * d10, cy = d3 * d10 - d2 * d11, cz = d2 * d7 - d3 * d6;
   e.V = (d1 * cx + d5 * cy + d9 * cz) / 6.0;
@@ -422,6 +432,30 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
     for (int64_t g = 0; g < N; ++g)
       for (int d = 0; d < 3; ++d) h->kd_host[4 * g + d] = (float)kdc[3 * L.node_of_internal[g] + d];
   }
+  // One Poisson ratio per tile: D of a tet is then E |V| times constants of
+  // the tile, so the CG's slot records fold sqrt(E |V|) into the gradients
+  // and take the constants from the tile (44 instead of 56 bytes per slot).
+  {
+    const int64_t nt = (int64_t)L.tiles.size();
+    std::vector<double4> td(nt, make_double4(0.0, 0.0, 0.0, 0.0));
+    bool one = true;
+    for (int64_t i = 0; i < nt && one; ++i) {
+      const TileInfo &ti = L.tiles[i];
+      if (ti.n_slots == 0) continue;
+      const int64_t t0 = L.slot_tet[ti.slot_begin] & 0x7fffffff;
+      for (int64_t s = ti.slot_begin; s < ti.slot_begin + ti.n_slots && one; ++s) {
+        const int64_t t = L.slot_tet[s] & 0x7fffffff;
+        one = h->poisson[t] == h->poisson[t0];
+      }
+      double dm[3];
+      material_d(1.0, h->poisson[t0], dm);
+      td[i] = make_double4(dm[0], dm[1], dm[2], 0.0);
+    }
+    h->one_nu_per_tile = one;
+    if (nt > 0)
+      HIPCHK(h, hipMemcpyAsync(h->tile_d, td.data(), nt * sizeof(double4), hipMemcpyHostToDevice,
+                               h->stream));
+  }
   std::vector<double> rest(9 * S), vd(3 * S);
   for (int64_t s = 0; s < S; ++s) {
     int64_t t = L.slot_tet[s] & 0x7fffffff;
@@ -492,6 +526,8 @@ StepArgs make_args(otfem_solver *h, double dt, int warp, const otfem_params &P) 
   a.store_rotations = P.store_rotations ? 1 : 0;
   a.plastic_on = h->plastic_on ? 1 : 0;
   a.store64 = (h->lc.spmv_mode == SPMV_F64_STORE64) ? 1 : 0;
+  a.tile_d = h->tile_d;
+  a.folded = (h->one_nu_per_tile && h->lc.spmv_mode == SPMV_F64) ? 1 : 0;
   return a;
 }
 

@@ -44,12 +44,15 @@ struct StepArgs {
   const int32_t *slot_tet;
   const double *rest64;  // [9][S]  grad N_k (k = 1..3) at rest
   const double *vd64;    // [3][S]  |V| * (D0, D1, D2)
-  float4 *bt4;           // [3][S]  (R grad N_k, |V| D_k), refreshed each step
+  float4 *bt4;           // [3][S]  (R grad N_k, |V| D_k), refreshed each step; with
+                         // `folded`, the same storage holds [2][S] float4 + [S] float:
+                         // the nine values sqrt(E |V|) R grad N_k (k = 1..3)
   double4 *bt8;          // [3][S]  same in FP64 (only with SPMV_F64_STORE64)
   // per tet (caller order)
   double *Re;            // [T][9]
   double *plastic;       // [T][6]
   const double *plastic_param;  // [T][3] yield, creep, max
+  const double4 *tile_d; // [ntiles] (D0, D1, D2, 0) per unit volume and unit E of a tile's one nu
   // bodies / reductions (no atomics except done_count)
   const int32_t *body_tile_begin;
   BodyState *body;
@@ -81,6 +84,7 @@ struct StepArgs {
   int32_t store_rotations;
   int32_t plastic_on;
   int32_t store64;       // element kernel also writes bt8
+  int32_t folded;        // slot records in folded form (every tile has one Poisson ratio)
 };
 
 struct LaunchConfig {

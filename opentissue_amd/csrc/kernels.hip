@@ -41,6 +41,7 @@
 #include "kernels.h"
 
 #include <cmath>
+#include <type_traits>
 
 namespace otfem {
 
@@ -337,6 +338,7 @@ __global__ __launch_bounds__(kElemBlock) void k_element(StepArgs A) {
       fe[0][d] = 0.0;
       kv[0][d] = 0.0;
     }
+    double btr[3][3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       double t[3], w[3], rt[3], rw[3];
@@ -352,11 +354,25 @@ __global__ __launch_bounds__(kElemBlock) void k_element(StepArgs A) {
         kv[0][d] -= rw[d];
       }
       // rotated gradient and |V| D_k for the per-iteration SpMV
-      double bt[3];
-      matvec(R, b[k], bt);
+      matvec(R, b[k], btr[k]);
       const double vdk = k == 0 ? D0 : (k == 1 ? D1 : D2);
-      A.bt4[k * S + s] = make_float4((float)bt[0], (float)bt[1], (float)bt[2], (float)vdk);
-      if (A.store64) A.bt8[k * S + s] = make_double4(bt[0], bt[1], bt[2], vdk);
+      if (!A.folded)
+        A.bt4[k * S + s] = make_float4((float)btr[k][0], (float)btr[k][1], (float)btr[k][2],
+                                       (float)vdk);
+      if (A.store64) A.bt8[k * S + s] = make_double4(btr[k][0], btr[k][1], btr[k][2], vdk);
+    }
+    if (A.folded) {
+      // sqrt(E |V|) R grad N_k: K' p = |V| sigma_E(G) bt_k is quadratic in
+      // the gradients and D = E |V| c(nu), so E |V| = D0 / c0 folds into
+      // them and c(nu) comes from the tile.
+      const double sq = sqrt(D0 / A.tile_d[tile].x);
+      float4 *bc4 = A.bt4;
+      float *bc1 = reinterpret_cast<float *>(A.bt4 + 2 * S);
+      bc4[s] = make_float4((float)(sq * btr[0][0]), (float)(sq * btr[0][1]),
+                           (float)(sq * btr[0][2]), (float)(sq * btr[1][0]));
+      bc4[S + s] = make_float4((float)(sq * btr[1][1]), (float)(sq * btr[1][2]),
+                               (float)(sq * btr[2][0]), (float)(sq * btr[2][1]));
+      bc1[s] = (float)(sq * btr[2][2]);
     }
 
     const int32_t st = A.slot_tet[s];
@@ -524,38 +540,6 @@ __device__ __forceinline__ void record_stop(const StepArgs &A, BodyState *w, int
 
 namespace {
 
-// Slot data: three packed (R grad N_k, |V| D_k) records in FP32 or FP64.
-template <typename MT, typename ST>
-__device__ __forceinline__ void load_slot(const StepArgs &A, int64_t S, int s, MT bt[3][3],
-                                          MT D[3]);
-template <>
-__device__ __forceinline__ void load_slot<float, float>(const StepArgs &A, int64_t S, int s,
-                                                        float bt[3][3], float D[3]) {
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    float4 q = A.bt4[k * S + s];
-    bt[k][0] = q.x; bt[k][1] = q.y; bt[k][2] = q.z; D[k] = q.w;
-  }
-}
-template <>
-__device__ __forceinline__ void load_slot<double, float>(const StepArgs &A, int64_t S, int s,
-                                                         double bt[3][3], double D[3]) {
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    float4 q = A.bt4[k * S + s];
-    bt[k][0] = q.x; bt[k][1] = q.y; bt[k][2] = q.z; D[k] = q.w;
-  }
-}
-template <>
-__device__ __forceinline__ void load_slot<double, double>(const StepArgs &A, int64_t S, int s,
-                                                          double bt[3][3], double D[3]) {
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    double4 q = A.bt8[k * S + s];
-    bt[k][0] = q.x; bt[k][1] = q.y; bt[k][2] = q.z; D[k] = q.w;
-  }
-}
-
 // Raw slot record as loaded (conversion to the math type happens at use, so
 // several records can be in flight per thread at little register cost).
 template <typename ST>
@@ -569,6 +553,16 @@ template <>
 struct SlotRaw<double> {
   ushort4 c;
   double4 q[3];
+};
+// Tag of the folded records: nine floats sqrt(E |V|) R grad N_k (k = 1..3)
+// per slot, 44 bytes with the corners; D / (E |V|) comes from the tile's one
+// Poisson ratio.
+struct Folded {};
+template <>
+struct SlotRaw<Folded> {
+  ushort4 c;
+  float4 q[2];
+  float z;
 };
 
 template <typename ST>
@@ -587,9 +581,20 @@ __device__ __forceinline__ void load_raw<double>(const StepArgs &A, int64_t S, i
 #pragma unroll
   for (int k = 0; k < 3; ++k) r.q[k] = A.bt8[k * S + s];
 }
+template <>
+__device__ __forceinline__ void load_raw<Folded>(const StepArgs &A, int64_t S, int s,
+                                                 SlotRaw<Folded> &r) {
+  r.c = A.conn[s];
+  r.q[0] = A.bt4[s];
+  r.q[1] = A.bt4[S + s];
+  r.z = reinterpret_cast<const float *>(A.bt4 + 2 * S)[s];
+}
 
+// Gradients and D of one slot in the math type; `Dt` is the tile's D (used
+// by folded records only).
 template <typename MT, typename ST>
-__device__ __forceinline__ void unpack_raw(const SlotRaw<ST> &r, MT bt[3][3], MT D[3]) {
+__device__ __forceinline__ void unpack_raw(const SlotRaw<ST> &r, const MT Dt[3], MT bt[3][3],
+                                           MT D[3]) {
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     bt[k][0] = (MT)r.q[k].x;
@@ -597,6 +602,17 @@ __device__ __forceinline__ void unpack_raw(const SlotRaw<ST> &r, MT bt[3][3], MT
     bt[k][2] = (MT)r.q[k].z;
     D[k] = (MT)r.q[k].w;
   }
+}
+template <>
+__device__ __forceinline__ void unpack_raw<double, Folded>(const SlotRaw<Folded> &r,
+                                                           const double Dt[3], double bt[3][3],
+                                                           double D[3]) {
+  bt[0][0] = r.q[0].x; bt[0][1] = r.q[0].y; bt[0][2] = r.q[0].z;
+  bt[1][0] = r.q[0].w; bt[1][1] = r.q[1].x; bt[1][2] = r.q[1].y;
+  bt[2][0] = r.q[1].z; bt[2][1] = r.q[1].w; bt[2][2] = r.z;
+  D[0] = Dt[0];
+  D[1] = Dt[1];
+  D[2] = Dt[2];
 }
 
 // y_owned += (K' p)_e for one slot: G = sum_k (p_k - p_0) bt_k^T,
@@ -795,6 +811,13 @@ void k_cg_iter(StepArgs A, int it) {
     tile = xcd_remap(blockIdx.x, gridDim.x);
   }
   const TileInfo ti = A.tiles[tile];
+  MT Dt[3] = {(MT)0, (MT)0, (MT)0};
+  if (std::is_same<ST, Folded>::value) {
+    const double4 td = A.tile_d[tile];
+    Dt[0] = (MT)td.x;
+    Dt[1] = (MT)td.y;
+    Dt[2] = (MT)td.z;
+  }
   BodyState *bs = &A.body[ti.body];
   // exit if the body stopped in an earlier kernel (a stop recorded by this
   // kernel still needs this workgroup's prologue: v += alpha p)
@@ -884,7 +907,7 @@ void k_cg_iter(StepArgs A, int it) {
       for (int u = 0; u < U; ++u) {
         if (s + u * BLOCK < send) {
           MT bt[3][3], D[3];
-          unpack_raw<MT, ST>(ra[u], bt, D);
+          unpack_raw<MT, ST>(ra[u], Dt, bt, D);
           slot_spmv<MT, AT>(ra[u].c, bt, D, sp, sy, no);
         }
       }
@@ -896,7 +919,7 @@ void k_cg_iter(StepArgs A, int it) {
       for (int u = 0; u < U; ++u) {
         if (s + (U + u) * BLOCK < send) {
           MT bt[3][3], D[3];
-          unpack_raw<MT, ST>(rb[u], bt, D);
+          unpack_raw<MT, ST>(rb[u], Dt, bt, D);
           slot_spmv<MT, AT>(rb[u].c, bt, D, sp, sy, no);
         }
       }
@@ -1025,6 +1048,8 @@ hipError_t enqueue_mode(const StepArgs &a, int mode, size_t lds_cg, hipStream_t 
     case SPMV_F32_ACC64:
       return enqueue_iterations<BLOCK, float, double, float>(a, lds_cg, s, it0, it1, g);
     case SPMV_F64:
+      if (a.folded)
+        return enqueue_iterations<BLOCK, double, double, Folded>(a, lds_cg, s, it0, it1, g);
       return enqueue_iterations<BLOCK, double, double, float>(a, lds_cg, s, it0, it1, g);
     case SPMV_F64_STORE64:
       return enqueue_iterations<BLOCK, double, double, double>(a, lds_cg, s, it0, it1, g);

@@ -120,6 +120,40 @@ def test_spmv_precision_modes(mode, tol):
     assert errs.max() <= tol, errs
 
 
+@pytest.mark.parametrize("materials", ["uniform", "E per tet", "E and nu per tet"])
+def test_slot_record_forms(materials):
+    """One Poisson ratio per tile lets the CG's slot records fold E |V| into
+    the rotated gradients (44 bytes per slot, D / (E |V|) per tile); a
+    Poisson ratio that varies inside a tile keeps D per slot (56 bytes).
+    Both match the oracle."""
+    from opentissue_amd import fem, mesh
+    import oracle
+    c, t, fx = _lattice(6, 10, 4)
+    rng = np.random.default_rng(3)
+    E, nu = np.full(len(t), 3e5), np.full(len(t), 0.3)
+    if materials != "uniform":
+        E = np.exp(rng.uniform(np.log(1e5), np.log(1e6), len(t)))
+    if materials == "E and nu per tet":
+        nu = rng.uniform(0.2, 0.45, len(t))
+    rho = np.full(len(t), 1000.0)
+    s = fem.Solver(c, t, fx)
+    s.init_materials(E, nu, rho)
+    o = oracle.Oracle(c, t, fx)
+    o.init_materials(E, nu, rho)
+    f = mesh.gravity_forces(np.where(fx == 1, 0.0, o.mass()), fx)
+    s.set_external_forces(f)
+    o.set_external_forces(f)
+    # CG runs to convergence (heterogeneous E and nu need more than the
+    # default cap of 200 iterations; a capped, unconverged step is not
+    # comparable at 1e-6), then the reference's fixed 20 iterations
+    for gp, op in ((fem.config_params(cap=3000), oracle.config_params(cap=3000)),
+                   (fem.reference_params(), oracle.reference_params())):
+        errs, iters = _per_step_errors(s, o, c, 3, 0.01, gp, op)
+        print(materials, errs, iters)
+        assert all(ig < 3000 for ig, _ in iters) or gp.cg_max_iterations == 20
+        assert errs.max() <= TOL, errs
+
+
 def test_rotations_match_oracle():
     from opentissue_amd import fem
     import oracle
