@@ -53,7 +53,6 @@ struct otfem_solver {
   double4 *tile_d = nullptr;  // [ntiles] D per unit volume and unit E of a tile's one nu
   bool one_nu_per_tile = false;  // slot records may use the folded form
   double *Re = nullptr, *plastic = nullptr, *plastic_param = nullptr;
-  int32_t *body_tile_begin = nullptr;
   BodyState *body = nullptr;
   double *part_bb = nullptr, *part_rr = nullptr, *part4 = nullptr;
   int *done_count = nullptr;  // [2]: stopped bodies, active tiles
@@ -138,7 +137,7 @@ void free_device(otfem_solver *h) {
   dfree(h->tiles); dfree(h->halo); dfree(h->conn); dfree(h->slot_tet);
   dfree(h->rest64); dfree(h->vd64); dfree(h->bt4); dfree(h->bt8); dfree(h->tile_d);
   dfree(h->Re); dfree(h->plastic); dfree(h->plastic_param);
-  dfree(h->body_tile_begin); dfree(h->body); dfree(h->part_bb); dfree(h->part_rr);
+  dfree(h->body); dfree(h->part_bb); dfree(h->part_rr);
   dfree(h->part4); dfree(h->done_count); dfree(h->iter_acc); dfree(h->unconv_acc);
   dfree(h->vw); dfree(h->hist1); dfree(h->hist2); dfree(h->kd); dfree(h->free_rank);
   dfree(h->ws_bad);
@@ -260,7 +259,6 @@ otfem_status ensure_device(otfem_solver *h) {
   HIPCHK(h, dalloc(&h->Re, 9 * T));
   HIPCHK(h, dalloc(&h->plastic, 6 * T));
   HIPCHK(h, dalloc(&h->plastic_param, 3 * T));
-  HIPCHK(h, dalloc(&h->body_tile_begin, B + 1));
   HIPCHK(h, dalloc(&h->body, B));
   HIPCHK(h, dalloc(&h->part_bb, nt));
   HIPCHK(h, dalloc(&h->part_rr, nt));
@@ -278,8 +276,6 @@ otfem_status ensure_device(otfem_solver *h) {
     HIPCHK(h, hipMemcpyAsync(h->conn, L.slot_conn.data(), S * sizeof(ushort4), hipMemcpyHostToDevice, s));
     HIPCHK(h, hipMemcpyAsync(h->slot_tet, L.slot_tet.data(), S * sizeof(int32_t), hipMemcpyHostToDevice, s));
   }
-  HIPCHK(h, hipMemcpyAsync(h->body_tile_begin, L.body_tile_begin.data(), (B + 1) * sizeof(int32_t),
-                           hipMemcpyHostToDevice, s));
   std::vector<double> tmp;
   to_internal3(L, h->x_init.data(), tmp);
   HIPCHK(h, hipMemcpyAsync(h->x, tmp.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice, s));
@@ -503,7 +499,7 @@ StepArgs make_args(otfem_solver *h, double dt, int warp, const otfem_params &P) 
   a.tiles = h->tiles; a.halo = h->halo; a.conn = h->conn; a.slot_tet = h->slot_tet;
   a.rest64 = h->rest64; a.vd64 = h->vd64; a.bt4 = h->bt4; a.bt8 = h->bt8;
   a.Re = h->Re; a.plastic = h->plastic; a.plastic_param = h->plastic_param;
-  a.body_tile_begin = h->body_tile_begin; a.body = h->body;
+  a.body = h->body;
   a.part_bb = h->part_bb; a.part_rr = h->part_rr; a.part4 = h->part4;
   a.done_count = h->done_count; a.iter_acc = h->iter_acc; a.unconv_acc = h->unconv_acc;
   a.act = h->act;

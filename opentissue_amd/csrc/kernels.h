@@ -54,7 +54,6 @@ struct StepArgs {
   const double *plastic_param;  // [T][3] yield, creep, max
   const double4 *tile_d; // [ntiles] (D0, D1, D2, 0) per unit volume and unit E of a tile's one nu
   // bodies / reductions (no atomics except done_count)
-  const int32_t *body_tile_begin;
   BodyState *body;
   double *part_bb;       // [ntiles]  b.b (element kernel)
   double *part_rr;       // [ntiles]  r_0.r_0 (element kernel)

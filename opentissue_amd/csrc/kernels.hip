@@ -85,15 +85,6 @@ __device__ __forceinline__ double block_sum(double a, double *red) {
   return r;
 }
 
-// Sum of one body's tile partials in tile order; identical in every
-// workgroup that evaluates it.
-template <int BLOCK>
-__device__ __forceinline__ double body_sum(const double *part, int t0, int t1, double *red) {
-  double s = 0.0;
-  for (int t = t0 + (int)threadIdx.x; t < t1; t += BLOCK) s += part[t];
-  return block_sum<BLOCK>(s, red);
-}
-
 // Four sums at once (one tree, one set of barriers); result broadcast.
 // `red4` holds 4 * (BLOCK/64 + 1) doubles of LDS.
 template <int BLOCK>
@@ -667,6 +658,10 @@ struct CgShape {
   static constexpr int kNodes = BLOCK <= 256 ? 4 : (BLOCK <= 512 ? 2 : 1);
   static constexpr int kSlots = BLOCK <= 256 ? 2 : 1;
   static constexpr int kWaves = BLOCK <= 256 ? 2 : 4;
+  // the stop test waits for nothing else and the body's tile partials are
+  // loaded with the other early loads (with 128 registers per thread: stop
+  // test first, partials after the node inputs)
+  static constexpr bool kEarlyPartials = BLOCK <= 256;
 };
 
 // One tile node's inputs to the CG prologue, as loaded.
@@ -819,16 +814,27 @@ void k_cg_iter(StepArgs A, int it) {
     Dt[2] = (MT)td.z;
   }
   BodyState *bs = &A.body[ti.body];
-  // exit if the body stopped in an earlier kernel (a stop recorded by this
-  // kernel still needs this workgroup's prologue: v += alpha p)
+  // Issued before any of them is used: the body's stop record, the first
+  // tile partials of the body sums, the first slot records and the first
+  // batch of node inputs (b.b is read after the sums: loading it early
+  // measured slower).
   const int st = *(volatile int32_t *)&bs->stopped;
-  if (st != 0 && st - 1 < it) return;
-  const int t0 = A.body_tile_begin[ti.body], t1 = A.body_tile_begin[ti.body + 1];
+  if (!CgShape<BLOCK>::kEarlyPartials && st != 0 && st - 1 < it) return;
+  const int t0 = ti.body_tile_begin, t1 = ti.body_tile_end;
   const int nt = A.ntiles;
   const int nn = ti.n_owned + ti.n_halo, no = ti.n_owned;
   const int cur = it & 1, prv = (it - 1) & 1;
   double *Rn = A.r + cur * 3 * A.N, *Pn = A.p + cur * 3 * A.N, *Yn = A.y + cur * 3 * A.N;
   const double *Rp = A.r + prv * 3 * A.N, *Pp = A.p + prv * 3 * A.N, *Yp = A.y + prv * 3 * A.N;
+  // tile partials: r.r and b.b of the element kernel at it == 0, else
+  // p.q, r.q, q.q, r.r of the previous kernel
+  const double4 *prevp = reinterpret_cast<const double4 *>(A.part4 + prv * 4 * (int64_t)nt);
+  auto partial = [&](int t) {
+    return it == 0 ? make_double4(A.part_rr[t], A.part_bb[t], 0.0, 0.0) : prevp[t];
+  };
+  double4 part = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (CgShape<BLOCK>::kEarlyPartials && t0 + (int)threadIdx.x < t1)
+    part = partial(t0 + (int)threadIdx.x);
 
   // first slot records; loads past the tile's last slot re-read that slot
   const int64_t S = A.S;
@@ -843,13 +849,27 @@ void k_cg_iter(StepArgs A, int it) {
   NodeIn nd[NB];
   load_nodes<BLOCK, NB>(A, ti, it, 0, Rp, Pp, Yp, nd);
 
+  // exit if the body stopped in an earlier kernel (a stop recorded by this
+  // kernel still needs this workgroup's prologue: v += alpha p)
+  if (st != 0 && st - 1 < it) return;
+  if (!CgShape<BLOCK>::kEarlyPartials && t0 + (int)threadIdx.x < t1)
+    part = partial(t0 + (int)threadIdx.x);
+
+  // body sums in tile order (bodies with more tiles than threads add the
+  // rest here); every workgroup of the body gets the same values
+  for (int t = t0 + (int)threadIdx.x + BLOCK; t < t1; t += BLOCK) {
+    const double4 q = partial(t);
+    part.x += q.x;
+    part.y += q.y;
+    part.z += q.z;
+    part.w += q.w;
+  }
+  const double4 s4 = block_sum4<BLOCK>(part, red);
   double alpha = 0.0, beta = 0.0, rr, bb;
   if (it == 0) {
-    rr = body_sum<BLOCK>(A.part_rr, t0, t1, red);
-    bb = body_sum<BLOCK>(A.part_bb, t0, t1, red);
+    rr = s4.x;
+    bb = s4.y;
   } else {
-    const double *prevp = A.part4 + prv * 4 * (int64_t)nt;
-    const double4 s4 = body_sum4<BLOCK>(reinterpret_cast<const double4 *>(prevp), t0, t1, red);
     double pq = s4.x;
     const double rq = s4.y, qq = s4.z, rrd = s4.w;
     bb = bs->bb;
@@ -948,7 +968,7 @@ __global__ __launch_bounds__(BLOCK) void k_finish(StepArgs A) {
   const int stopped = *(volatile int32_t *)&bs->stopped;
   const int it = A.cg_max;
   if (!stopped && it > 0) {
-    const int t0 = A.body_tile_begin[ti.body], t1 = A.body_tile_begin[ti.body + 1];
+    const int t0 = ti.body_tile_begin, t1 = ti.body_tile_end;
     const double *prevp = A.part4 + ((it - 1) & 1) * 4 * (int64_t)A.ntiles;
     const double4 s4 = body_sum4<BLOCK>(reinterpret_cast<const double4 *>(prevp), t0, t1, red);
     double pq = s4.x;

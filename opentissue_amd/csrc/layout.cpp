@@ -145,6 +145,10 @@ std::string build_layout(int64_t N, int64_t T, const double *coords, const int32
     }
   }
   L.body_tile_begin[L.B] = (int32_t)L.tiles.size();
+  for (auto &ti : L.tiles) {
+    ti.body_tile_begin = L.body_tile_begin[ti.body];
+    ti.body_tile_end = L.body_tile_begin[ti.body + 1];
+  }
   const int64_t ntiles = (int64_t)L.tiles.size();
 
   // 2. Slots: each tet goes into every distinct tile owning one of its corners.

@@ -28,6 +28,8 @@ struct TileInfo {
   int32_t n_slots;
   int32_t body;
   int32_t body_rank;   // index of this tile within its body
+  int32_t body_tile_begin;  // tiles [body_tile_begin, body_tile_end) are the body's
+  int32_t body_tile_end;
 };
 
 struct Layout {
