@@ -431,9 +431,10 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
   // One Poisson ratio per tile: D of a tet is then E |V| times constants of
   // the tile, so the CG's slot records fold sqrt(E |V|) into the gradients
   // and take the constants from the tile (44 instead of 56 bytes per slot).
+  // (td outlives the stream synchronisation below: the copy is asynchronous)
+  std::vector<double4> td(L.tiles.size(), make_double4(0.0, 0.0, 0.0, 0.0));
   {
     const int64_t nt = (int64_t)L.tiles.size();
-    std::vector<double4> td(nt, make_double4(0.0, 0.0, 0.0, 0.0));
     bool one = true;
     for (int64_t i = 0; i < nt && one; ++i) {
       const TileInfo &ti = L.tiles[i];
