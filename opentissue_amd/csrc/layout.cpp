@@ -105,6 +105,33 @@ std::string build_layout(int64_t N, int64_t T, const double *coords, const int32
   std::vector<double> weight(N, 0.0);
   for (int64_t q = 0; q < 4 * T; ++q) weight[tets[q]] += 1.0;
   for (auto &wv : weight) wv = wv * wv * wv + 1.0;
+  // The bisection runs on smoothed coordinates (each pass moves a node to the
+  // mean centroid of its tets): on a jittered lattice, cuts through the raw
+  // coordinates are ragged and cut more tets (64^3 cube, jitter 0.2 h: 1.43
+  // slots per tet; after 8 passes 1.37, against 1.35 without jitter).  Nodes
+  // of no tet keep their coordinates.
+  std::vector<double> xs(coords, coords + 3 * N);
+  {
+    std::vector<double> acc(3 * N);
+    std::vector<int32_t> cnt(N);
+    for (int pass = 0; pass < 8 && T > 0; ++pass) {
+      std::fill(acc.begin(), acc.end(), 0.0);
+      std::fill(cnt.begin(), cnt.end(), 0);
+      for (int64_t t = 0; t < T; ++t) {
+        const int32_t *tt = tets + 4 * t;
+        double c[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < 4; ++k)
+          for (int d = 0; d < 3; ++d) c[d] += xs[3 * (int64_t)tt[k] + d];
+        for (int k = 0; k < 4; ++k) {
+          for (int d = 0; d < 3; ++d) acc[3 * (int64_t)tt[k] + d] += 0.25 * c[d];
+          ++cnt[tt[k]];
+        }
+      }
+      for (int64_t i = 0; i < N; ++i)
+        if (cnt[i] > 0)
+          for (int d = 0; d < 3; ++d) xs[3 * i + d] = acc[3 * i + d] / cnt[i];
+    }
+  }
   L.node_of_internal.resize(N);
   L.internal_of_node.assign(N, -1);
   std::vector<int32_t> tile_of_internal(N);
@@ -121,7 +148,7 @@ std::string build_layout(int64_t N, int64_t T, const double *coords, const int32
       if (k >= 1) parts = k * cus;
     }
     std::vector<int32_t> sizes;
-    rcb(idx, 0, (int64_t)idx.size(), parts, coords, weight.data(), sizes);
+    rcb(idx, 0, (int64_t)idx.size(), parts, xs.data(), weight.data(), sizes);
     int64_t pos = 0;
     L.body_tile_begin[b] = (int32_t)L.tiles.size();
     int32_t rank = 0;
