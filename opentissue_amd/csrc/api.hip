@@ -9,6 +9,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -63,7 +64,9 @@ struct otfem_solver {
   // captured step graphs (element part, CG part) and the inputs they bake in
   bool use_graphs = true;
   hipGraphExec_t g_elem = nullptr, g_fin = nullptr;
-  std::map<std::pair<int, int>, hipGraphExec_t> g_chunks;  // (chunk, grid) -> CG iterations
+  // (first iteration, iterations, grid) -> captured CG iterations
+  std::map<std::tuple<int, int, int>, hipGraphExec_t> g_chunks;
+  int cg_stop_seen = 0;  // end of the chunk in which the last step's CG was seen to stop
                                                           // [j C, (j+1) C), built on first use
   hipStream_t cap_stream = nullptr;      // stream used only for graph capture
   int *done_host = nullptr;              // [2][2] pinned copies of done_count
@@ -120,6 +123,7 @@ void dfree(T *&p) {
 }
 
 constexpr int kCgChunk = 32;  // CG iterations per captured graph
+constexpr int kCgTail = 8;    // iterations per graph near the expected stop
 
 void drop_graphs(otfem_solver *h) {
   if (h->g_elem) (void)hipGraphExecDestroy(h->g_elem);
@@ -880,7 +884,6 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
   if (ws_on) HIPCHK(h, enqueue_ws_predict(w, s));
   if (timed) HIPCHK(h, hipEventRecord(e[1], s));
   const int cap = std::max(0, P.cg_max_iterations);
-  const int nchunks = (cap + kCgChunk - 1) / kCgChunk;
   // capture `enqueue` (on the capture-only stream) into an executable graph
   auto capture = [&](auto &&enqueue, hipGraphExec_t *out, const char *what) -> otfem_status {
     hipGraph_t g;
@@ -917,7 +920,9 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
     HIPCHK(h, enqueue_element(a, h->lds_elem, s));
   }
   if (timed) HIPCHK(h, hipEventRecord(e[2], s));
-  // CG in chunks of kCgChunk iterations.  The host keeps one chunk queued
+  // CG in chunks: kCgChunk iterations, and kCgTail from 16 iterations before
+  // the iteration at which the previous step was seen to stop (so the chunks
+  // that run after the stop are short).  The host keeps one chunk queued
   // ahead of the one whose stopped-body count it reads back, and issues no
   // more once every body has stopped; kernels of a chunk that runs after the
   // stop exit at once.
@@ -927,8 +932,11 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
   // times 256, capped at the tile count.
   const int ntiles = (int)h->L.tiles.size();
   int known_active = ntiles;
-  for (int j = 0; j < nchunks; ++j) {
-    const int it0 = j * kCgChunk, it1 = std::min(cap, it0 + kCgChunk);
+  const int tail_from = h->cg_stop_seen > 0 ? std::max(0, h->cg_stop_seen - 16) : cap;
+  int prev_end = 0;
+  for (int j = 0, it0 = 0; it0 < cap; ++j) {
+    const int len = (it0 % kCgChunk == 0 && it0 + kCgChunk <= tail_from) ? kCgChunk : kCgTail;
+    const int it1 = std::min(cap, it0 + len);
     int grid = ntiles;
     if (a.compact) {
       grid = 256;
@@ -936,7 +944,7 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
       grid = std::min(grid, ntiles);
     }
     if (h->use_graphs) {
-      hipGraphExec_t &ge = h->g_chunks[std::make_pair(j, grid)];
+      hipGraphExec_t &ge = h->g_chunks[std::make_tuple(it0, it1 - it0, grid)];
       if (!ge) {
         otfem_status cs = capture(
             [&](hipStream_t cs_) { return enqueue_cg(a, h->lc, h->lds_cg, cs_, it0, it1, grid); },
@@ -947,18 +955,23 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
     } else {
       HIPCHK(h, enqueue_cg(a, h->lc, h->lds_cg, s, it0, it1, grid));
     }
-    if (j + 1 < nchunks) {
+    if (it1 < cap) {
       HIPCHK(h, hipMemcpyAsync(&h->done_host[2 * (j & 1)], h->done_count, 2 * sizeof(int),
                                hipMemcpyDeviceToHost, s));
       HIPCHK(h, hipEventRecord(h->chunk_ev[j & 1], s));
       if (j >= 1) {
         HIPCHK(h, hipEventSynchronize(h->chunk_ev[(j - 1) & 1]));
         const volatile int *dh = &h->done_host[2 * ((j - 1) & 1)];
-        if (dh[0] >= (int)h->L.B) break;
+        if (dh[0] >= (int)h->L.B) {
+          h->cg_stop_seen = prev_end;
+          break;
+        }
         const int act_now = dh[1];
         if (a.compact) known_active = std::min(known_active, std::max(act_now, 1));
       }
     }
+    prev_end = it1;
+    it0 = it1;
   }
   if (h->use_graphs)
     HIPCHK(h, hipGraphLaunch(h->g_fin, s));
