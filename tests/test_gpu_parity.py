@@ -154,6 +154,25 @@ def test_slot_record_forms(materials):
         assert errs.max() <= TOL, errs
 
 
+@pytest.mark.parametrize("mode", ["ref", "cfg"])
+def test_irregular_mesh_read_from_xml(mode):
+    """An irregular (TetGen-made) mesh entering through the T4MESH XML reader:
+    the reference's fixture demos/data/xml/blocky.xml (438 nodes, 1,831 tets),
+    fixed at its lowest y, under gravity along -z."""
+    import os
+    from opentissue_amd import fem, mesh_io
+    import oracle
+    c, t = mesh_io.xml_read(os.path.join(os.path.dirname(__file__), "golden", "blocky.xml"))
+    fx = (c[:, 1] < c[:, 1].min() + 1e-9).astype(np.uint8)
+    assert 0 < fx.sum() < len(fx)
+    s, o = _pair(c, t, fx)
+    gp = fem.reference_params() if mode == "ref" else fem.config_params(cap=3000)
+    op = oracle.reference_params() if mode == "ref" else oracle.config_params(cap=3000)
+    errs, iters = _per_step_errors(s, o, c, 5, 0.01, gp, op)
+    print(mode, errs, iters)
+    assert errs.max() <= TOL, errs
+
+
 def test_rotations_match_oracle():
     from opentissue_amd import fem
     import oracle

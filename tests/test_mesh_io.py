@@ -1,0 +1,150 @@
+"""T4MESH XML read / write and the TetGen reader (CPU): the Python readers
+(opentissue_amd/mesh_io.py) and the C++ header (include/otfem/t4mesh_io.hpp,
+through tests/cpp/mesh_io_check) against the reference's fixture
+demos/data/xml/blocky.xml, against an independent parse of it
+(tests/golden/blocky.npz, regular expressions), and on round trips."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from opentissue_amd import mesh, mesh_io
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BLOCKY = os.path.join(HERE, "golden", "blocky.xml")
+CHECK = os.path.join(HERE, "cpp", "build", "mesh_io_check")
+
+
+def _golden():
+    d = np.load(os.path.join(HERE, "golden", "blocky.npz"))
+    return d["coords"], d["tets"]
+
+
+def _cpp(*args):
+    assert os.path.exists(CHECK), "build() compiles tests/cpp/mesh_io_check.cpp"
+    r = subprocess.run([CHECK, *args], capture_output=True, text=True, timeout=60)
+    return r.returncode, r.stdout
+
+
+def _checksums(coords, tets):
+    n = np.arange(1, coords.shape[0] + 1)[:, None] * np.array([1.0, 2.0, 3.0])
+    t = np.arange(1, tets.shape[0] + 1, dtype=np.uint64)[:, None] * np.arange(1, 5, dtype=np.uint64)
+    return (coords.sum(), (n * coords).sum(), int(tets.astype(np.uint64).sum()),
+            int((t * tets.astype(np.uint64)).sum()))
+
+
+def _parse_report(line):
+    f = line.split()
+    return dict(nodes=int(f[2]), tets=int(f[4]), coord_sum=float(f[6]), coord_wsum=float(f[8]),
+                index_sum=int(f[10]), index_wsum=int(f[12]))
+
+
+def test_blocky_fixture_matches_independent_parse():
+    coords, tets = mesh_io.xml_read(BLOCKY)
+    gc, gt = _golden()
+    assert coords.shape == (438, 3) and tets.shape == (1831, 4)
+    assert np.array_equal(tets, gt)
+    assert np.array_equal(coords, gc)
+    # a TetGen-made mesh: every tet positively oriented
+    assert (mesh.tet_volumes(coords, tets) > 0).all()
+
+
+def test_xml_write_read_round_trip_is_exact(tmp_path):
+    c, t = mesh.generate_blocks(3, 4, 2, 0.01, 0.02, 0.03)
+    c = mesh.jitter_interior(c, t, 0.01, 0.2, 5)
+    p = str(tmp_path / "m.xml")
+    mesh_io.xml_write(p, c, t)
+    c2, t2 = mesh_io.xml_read(p)
+    assert np.array_equal(c, c2) and np.array_equal(t, t2)
+
+
+def test_cpp_reader_and_writer_agree_with_python(tmp_path):
+    out = str(tmp_path / "cpp.xml")
+    rc, stdout = _cpp(BLOCKY, out)
+    assert rc == 0, stdout
+    rep = _parse_report(stdout.splitlines()[0])
+    gc, gt = _golden()
+    cs, cw, ts, tw = _checksums(gc, gt)
+    assert rep["nodes"] == 438 and rep["tets"] == 1831
+    assert rep["index_sum"] == ts and rep["index_wsum"] == tw
+    assert rep["coord_sum"] == pytest.approx(cs, rel=1e-13, abs=1e-12)
+    assert rep["coord_wsum"] == pytest.approx(cw, rel=1e-13, abs=1e-12)
+    # the C++ writer's file reads back to the fixture exactly (17 digits)
+    c2, t2 = mesh_io.xml_read(out)
+    assert np.array_equal(c2, gc) and np.array_equal(t2, gt)
+
+
+def _write_tetgen(base, coords, tets, first=1, node_attr=0, boundary=False, region=False):
+    with open(base + ".node", "w") as f:
+        f.write(f"# nodes\n{len(coords)} 3 {node_attr} {1 if boundary else 0}\n")
+        for i, p in enumerate(coords):
+            extra = " 0.5" * node_attr + (" 1" if boundary else "")
+            f.write(f"{i + first} {float(p[0])!r} {float(p[1])!r} {float(p[2])!r}{extra}\n")
+        f.write("# Generated by a test\n")
+    with open(base + ".ele", "w") as f:
+        f.write(f"{len(tets)} 4 {1 if region else 0}\n")
+        for k, t in enumerate(tets):
+            f.write(f"{k + first} " + " ".join(str(int(v) + first) for v in t) + (" 7" if region else "") + "\n")
+
+
+@pytest.mark.parametrize("first,node_attr,boundary,region", [(1, 0, False, False), (0, 2, True, True)])
+def test_tetgen_read(tmp_path, first, node_attr, boundary, region):
+    gc, gt = _golden()
+    base = str(tmp_path / "blocky")
+    _write_tetgen(base, gc, gt, first, node_attr, boundary, region)
+    c, t = mesh_io.tetgen_read(base)
+    assert np.array_equal(c, gc) and np.array_equal(t, gt)
+    rc, stdout = _cpp(BLOCKY, str(tmp_path / "o.xml"), base)
+    assert rc == 0, stdout
+    rep = _parse_report(stdout.splitlines()[1])
+    cs, cw, ts, tw = _checksums(gc, gt)
+    assert (rep["nodes"], rep["tets"], rep["index_sum"], rep["index_wsum"]) == (438, 1831, ts, tw)
+    assert rep["coord_sum"] == pytest.approx(cs, rel=1e-13, abs=1e-12)
+
+
+def test_tetgen_hand_written(tmp_path):
+    base = str(tmp_path / "one")
+    with open(base + ".node", "w") as f:
+        f.write("4  3  0  0\n   10 0 0 0\n   11 1 0 0\n   12 0 1 0\n   13 0 0 1\n")
+    with open(base + ".ele", "w") as f:
+        f.write("1 4 0\n  5  10 11 12 13  # one tet\n")
+    c, t = mesh_io.tetgen_read(base)
+    assert np.array_equal(c, np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]]))
+    assert np.array_equal(t, np.array([[0, 1, 2, 3]]))
+
+
+BAD_XML = {
+    "missing m": '<T4MESH nodes="4"><TETRAHEDRON i="0" j="1" k="2"/></T4MESH>',
+    "index out of range": '<T4MESH nodes="4"><TETRAHEDRON i="0" j="1" k="2" m="4"/></T4MESH>',
+    "point out of range": '<T4MESH nodes="4"><POINT idx="4" coord="[0,0,0]"/></T4MESH>',
+    "no node count": '<T4MESH><TETRAHEDRON i="0" j="1" k="2" m="3"/></T4MESH>',
+    "no T4MESH": '<MESH nodes="4"/>',
+    "missing coord": '<T4MESH nodes="4"><POINT idx="1"/></T4MESH>',
+    "non-finite coord": '<T4MESH nodes="4"><POINT idx="1" coord="[0,nan,0]"/></T4MESH>',
+}
+
+
+@pytest.mark.parametrize("case", sorted(BAD_XML))
+def test_xml_errors(tmp_path, case):
+    p = str(tmp_path / "bad.xml")
+    with open(p, "w") as f:
+        f.write(BAD_XML[case])
+    with pytest.raises(ValueError):
+        mesh_io.xml_read(p)
+    rc, _ = _cpp(p, str(tmp_path / "o.xml"))
+    assert rc == 1
+
+
+def test_tetgen_errors(tmp_path):
+    base = str(tmp_path / "e")
+    with open(base + ".node", "w") as f:
+        f.write("4 3 0 0\n1 0 0 0\n2 1 0 0\n3 0 1 0\n4 0 0 1\n")
+    with open(base + ".ele", "w") as f:
+        f.write("1 10 0\n1 1 2 3 4 5 6 7 8 9 10\n")
+    with pytest.raises(ValueError):
+        mesh_io.tetgen_read(base)
+    with open(base + ".ele", "w") as f:
+        f.write("1 4 0\n1 1 2 3 9\n")
+    with pytest.raises(ValueError):
+        mesh_io.tetgen_read(base)
