@@ -271,6 +271,36 @@ otfem_status otfem_set_warmstart_model(otfem_handle h, otfem_model_handle m);
  * store fails the step). */
 otfem_status otfem_set_recording(otfem_handle h, int on);
 
+/* ---- Surface coupling (skinning a render surface onto the tetrahedra) ---- */
+
+/* t4mesh::mesh_coupling::bind_surface (t4mesh_mesh_coupling.h:100-200): for
+ * each of `num_vertices` surface vertices vertices[3n] (caller frame), a
+ * tetrahedron containing it at the solver's current node positions -- all
+ * barycentric coordinates in (-1e-4, 1 + 1e-4) -- as tet_index[n] (caller
+ * tetrahedron order; the lowest such index; -1 if none) and the coordinates
+ * of its corners j, k, m as barycentric[3n].  Host computation (uniform grid
+ * over the tetrahedra's bounding boxes). */
+otfem_status otfem_surface_bind(otfem_handle h, int64_t num_vertices, const double *vertices,
+                                double *barycentric, int32_t *tet_index);
+
+typedef struct otfem_surface *otfem_surface_handle;
+/* A bound surface on the device (as otfem_surface_bind returns it, or any
+ * caller-made binding); vertices[3n] are kept for unbound vertices
+ * (tet_index -1), which otfem_surface_update leaves where they are.  The
+ * surface refers to the solver: destroy it first.  No reference counterpart
+ * (the reference updates surfaces on the host). */
+otfem_status otfem_surface_create(otfem_handle h, int64_t num_vertices, const double *vertices,
+                                  const double *barycentric, const int32_t *tet_index,
+                                  otfem_surface_handle *out);
+/* t4mesh::mesh_coupling::update_surface (t4mesh_mesh_coupling.h:202-238):
+ * every bound vertex at w0 p_i + w1 p_j + w2 p_k + w3 p_m (w0 = 1 - w1 - w2 -
+ * w3) of the solver's current node positions, computed on the device after
+ * the solver's queued steps, into vertices_out[3n] (caller frame).
+ * kernel_ms (may be NULL) receives the update kernel's time. */
+otfem_status otfem_surface_update(otfem_surface_handle s, double *vertices_out, double *kernel_ms);
+const char *otfem_surface_last_error(otfem_surface_handle s);
+void otfem_surface_destroy(otfem_surface_handle s);
+
 #ifdef __cplusplus
 }
 #endif

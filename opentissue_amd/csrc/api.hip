@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "kernels.h"
+#include "otfem/mesh_coupling.hpp"
 #include "layout.h"
 #include "mlp.h"
 #include "otfem/otfem.h"
@@ -1234,6 +1235,150 @@ otfem_status otfem_layout_info(otfem_handle h, int64_t *num_tiles, int64_t *num_
   if (num_halo) *num_halo = (int64_t)h->L.halo.size();
   if (max_tile_nodes) *max_tile_nodes = h->L.max_tile_nodes;
   return OTFEM_OK;
+}
+
+// ---- surface coupling ----
+
+otfem_status otfem_surface_bind(otfem_handle h, int64_t num_vertices, const double *vertices,
+                                double *barycentric, int32_t *tet_index) {
+  if (!h) return OTFEM_INVALID_ARG;
+  if (num_vertices < 0 || (num_vertices > 0 && (!vertices || !barycentric || !tet_index)))
+    return fail(h, OTFEM_INVALID_ARG, "surface arrays missing or negative vertex count");
+  otfem_status st = ensure_device(h);
+  if (st) return st;
+  // current node positions (the reference binds at node->m_coord), caller order
+  std::vector<double> tmp(3 * h->L.N), x(3 * h->L.N);
+  HIPCHK(h, hipMemcpyAsync(tmp.data(), h->x, tmp.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  to_caller3(h->L, tmp, x.data());
+  coupling::bind_points(h->L.N, x.data(), h->L.T, h->tets.data(), num_vertices, vertices,
+                        barycentric, tet_index);
+  return OTFEM_OK;
+}
+
+struct otfem_surface {
+  otfem_solver *solver = nullptr;
+  int64_t n = 0;
+  int4 *ids = nullptr;      // [n] internal node ids of the tetrahedron; x = -1: unbound
+  double4 *w = nullptr;     // [n] w0..w3
+  double *rest = nullptr;   // [3n] positions kept for unbound vertices
+  double *out = nullptr;    // [3n]
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+};
+
+namespace {
+otfem_status surface_fail(otfem_surface *s, otfem_status st, const std::string &msg) {
+  s->err = msg;
+  return st;
+}
+void surface_free(otfem_surface *s) {
+  if (s->ids) (void)hipFree(s->ids);
+  if (s->w) (void)hipFree(s->w);
+  if (s->rest) (void)hipFree(s->rest);
+  if (s->out) (void)hipFree(s->out);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+}
+}  // namespace
+
+#define SCHK(s, call)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return surface_fail((s), OTFEM_HIP_ERROR, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+otfem_status otfem_surface_create(otfem_handle h, int64_t num_vertices, const double *vertices,
+                                  const double *barycentric, const int32_t *tet_index,
+                                  otfem_surface_handle *out) {
+  if (!h || !out) return OTFEM_INVALID_ARG;
+  *out = nullptr;
+  if (num_vertices < 0 || (num_vertices > 0 && (!vertices || !barycentric || !tet_index)))
+    return fail(h, OTFEM_INVALID_ARG, "surface arrays missing or negative vertex count");
+  otfem_status st = ensure_device(h);
+  if (st) return st;
+  const int64_t n = num_vertices;
+  std::vector<int4> ids((size_t)n);
+  std::vector<double4> w((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t t = tet_index[i];
+    if (t < -1 || t >= h->L.T)
+      return fail(h, OTFEM_INVALID_ARG, "surface vertex " + std::to_string(i) +
+                                            ": tetrahedron index out of range");
+    if (t < 0) {
+      ids[(size_t)i] = make_int4(-1, -1, -1, -1);
+      w[(size_t)i] = make_double4(0.0, 0.0, 0.0, 0.0);
+      continue;
+    }
+    const int32_t *tt = &h->tets[4 * (int64_t)t];
+    ids[(size_t)i] = make_int4(h->L.internal_of_node[tt[0]], h->L.internal_of_node[tt[1]],
+                               h->L.internal_of_node[tt[2]], h->L.internal_of_node[tt[3]]);
+    const double w1 = barycentric[3 * i], w2 = barycentric[3 * i + 1], w3 = barycentric[3 * i + 2];
+    w[(size_t)i] = make_double4(1.0 - w1 - w2 - w3, w1, w2, w3);
+  }
+  auto *s = new otfem_surface();
+  s->solver = h;
+  s->n = n;
+  auto bail = [&](otfem_status e) {
+    fail(h, e, s->err);
+    surface_free(s);
+    delete s;
+    return e;
+  };
+  otfem_status e = OTFEM_OK;
+  auto step = [&](hipError_t r, const char *what) {
+    if (r != hipSuccess && e == OTFEM_OK) {
+      s->err = std::string(what) + ": " + hipGetErrorString(r);
+      e = OTFEM_HIP_ERROR;
+    }
+  };
+  if (n > 0) {
+    step(hipMalloc(&s->ids, n * sizeof(int4)), "hipMalloc");
+    step(hipMalloc(&s->w, n * sizeof(double4)), "hipMalloc");
+    step(hipMalloc(&s->rest, 3 * n * sizeof(double)), "hipMalloc");
+    step(hipMalloc(&s->out, 3 * n * sizeof(double)), "hipMalloc");
+    if (e == OTFEM_OK) {
+      step(hipMemcpy(s->ids, ids.data(), n * sizeof(int4), hipMemcpyHostToDevice), "hipMemcpy");
+      step(hipMemcpy(s->w, w.data(), n * sizeof(double4), hipMemcpyHostToDevice), "hipMemcpy");
+      step(hipMemcpy(s->rest, vertices, 3 * n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+    }
+  }
+  step(hipEventCreate(&s->ev0), "hipEventCreate");
+  step(hipEventCreate(&s->ev1), "hipEventCreate");
+  if (e != OTFEM_OK) return bail(e);
+  *out = s;
+  return OTFEM_OK;
+}
+
+otfem_status otfem_surface_update(otfem_surface_handle s, double *vertices_out, double *kernel_ms) {
+  if (!s) return OTFEM_INVALID_ARG;
+  if (s->n > 0 && !vertices_out)
+    return surface_fail(s, OTFEM_INVALID_ARG, "vertices_out is NULL");
+  otfem_solver *h = s->solver;
+  hipStream_t st = h->stream;
+  SCHK(s, hipEventRecord(s->ev0, st));
+  SCHK(s, enqueue_surface_update(s->n, s->ids, s->w, s->rest, h->x, s->out, st));
+  SCHK(s, hipEventRecord(s->ev1, st));
+  if (s->n > 0)
+    SCHK(s, hipMemcpyAsync(vertices_out, s->out, 3 * s->n * sizeof(double), hipMemcpyDeviceToHost, st));
+  SCHK(s, hipStreamSynchronize(st));
+  if (kernel_ms) {
+    float ms = 0.0f;
+    SCHK(s, hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    *kernel_ms = ms;
+  }
+  return OTFEM_OK;
+}
+
+const char *otfem_surface_last_error(otfem_surface_handle s) { return s ? s->err.c_str() : ""; }
+
+void otfem_surface_destroy(otfem_surface_handle s) {
+  if (!s) return;
+  if (s->solver && s->solver->stream) (void)hipStreamSynchronize(s->solver->stream);
+  surface_free(s);
+  delete s;
 }
 
 otfem_status otfem_set_warmstart_model(otfem_handle h, otfem_model_handle m) {

@@ -101,6 +101,10 @@ hipError_t enqueue_element(const StepArgs &a, size_t lds_elem, hipStream_t s);
 hipError_t enqueue_cg(const StepArgs &a, const LaunchConfig &lc, size_t lds_cg, hipStream_t s,
                       int it0, int it1, int grid);
 hipError_t enqueue_finish(const StepArgs &a, const LaunchConfig &lc, hipStream_t s);
+// Surface coupling: out[3i..] = sum_k w[i].k x[3 ids[i].k ..] for bound
+// vertices (ids[i].x >= 0, internal node ids), rest[3i..] for the others.
+hipError_t enqueue_surface_update(int64_t n, const int4 *ids, const double4 *w, const double *rest,
+                                  const double *x, double *out, hipStream_t s);
 size_t element_lds_bytes(int max_nodes, int max_owned, int plastic_on);
 size_t cg_lds_bytes(int max_nodes, int max_owned);
 

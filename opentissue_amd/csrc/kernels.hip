@@ -1099,6 +1099,48 @@ hipError_t enqueue_cg(const StepArgs &a, const LaunchConfig &lc, size_t lds_cg, 
   }
 }
 
+namespace {
+
+// t4mesh_mesh_coupling.h:224-235 per surface vertex: the barycentric
+// combination of its tetrahedron's four current node positions, summed in the
+// reference's order (p0 w0 + p1 w1) + p2 w2) + p3 w3.  HBM-bound gather.
+constexpr int kSurfaceBlock = 256;
+__global__ __launch_bounds__(kSurfaceBlock) void k_surface_update(int64_t n, const int4 *ids,
+                                                                  const double4 *w,
+                                                                  const double *rest,
+                                                                  const double *x, double *out) {
+  const int64_t i = (int64_t)blockIdx.x * kSurfaceBlock + threadIdx.x;
+  if (i >= n) return;
+  const int4 id = ids[i];
+  if (id.x < 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out[3 * i + d] = rest[3 * i + d];
+    return;
+  }
+  const double4 ww = w[i];
+  const double *p0 = x + 3 * (int64_t)id.x, *p1 = x + 3 * (int64_t)id.y;
+  const double *p2 = x + 3 * (int64_t)id.z, *p3 = x + 3 * (int64_t)id.w;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double v = p0[d] * ww.x;
+    v = v + p1[d] * ww.y;
+    v = v + p2[d] * ww.z;
+    v = v + p3[d] * ww.w;
+    out[3 * i + d] = v;
+  }
+}
+
+}  // namespace
+
+hipError_t enqueue_surface_update(int64_t n, const int4 *ids, const double4 *w, const double *rest,
+                                  const double *x, double *out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int64_t blocks = (n + kSurfaceBlock - 1) / kSurfaceBlock;
+  hipLaunchKernelGGL(k_surface_update, dim3((unsigned)blocks), dim3(kSurfaceBlock), 0, s, n, ids, w,
+                     rest, x, out);
+  return hipGetLastError();
+}
+
 hipError_t enqueue_finish(const StepArgs &a, const LaunchConfig &lc, hipStream_t s) {
   switch (lc.cg_block) {
     case 256: hipLaunchKernelGGL((k_finish<256>), dim3(a.ntiles), dim3(256), 0, s, a); break;

@@ -142,6 +142,12 @@ SIGNATURES = {
                                                 C.POINTER(C.c_int32), C.c_int, _dp, _fp]),
     "otfem_set_warmstart_model": (C.c_int, [_H, _M]),
     "otfem_set_recording": (C.c_int, [_H, C.c_int]),
+    "otfem_surface_bind": (C.c_int, [_H, C.c_int64, _dp, _dp, C.POINTER(C.c_int32)]),
+    "otfem_surface_create": (C.c_int, [_H, C.c_int64, _dp, _dp, C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_void_p)]),
+    "otfem_surface_update": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "otfem_surface_last_error": (C.c_char_p, [C.c_void_p]),
+    "otfem_surface_destroy": (None, [C.c_void_p]),
 }
 
 
@@ -377,6 +383,18 @@ class Solver:
             cols.ctypes.data_as(C.POINTER(C.c_int32)), _dptr(blocks)))
         return rp, cols, blocks
 
+    def surface_bind(self, vertices):
+        """t4mesh_mesh_coupling.h:100-200: (barycentric [n, 3], tet_index [n])
+        of surface vertices [n, 3] at the current node positions; index -1
+        where no tetrahedron contains the vertex."""
+        v = _f64(vertices, (-1, 3))
+        n = v.shape[0]
+        w = np.zeros((n, 3))
+        idx = np.zeros(n, dtype=np.int32)
+        self._check(self.lib.otfem_surface_bind(self.h, n, _dptr(v), _dptr(w),
+                                                idx.ctypes.data_as(C.POINTER(C.c_int32))))
+        return w, idx
+
     def layout_info(self):
         vals = [C.c_int64() for _ in range(4)]
         self._check(self.lib.otfem_layout_info(self.h, *[C.byref(v) for v in vals]))
@@ -395,6 +413,47 @@ class Solver:
 
 def _f32(a, shape):
     return np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(shape))
+
+
+class Surface:
+    """A bound surface on the device (``otfem_surface_*``): update() returns
+    the vertex positions from the solver's current node positions
+    (t4mesh_mesh_coupling.h:202-238).  Close it before the solver."""
+
+    def __init__(self, solver, vertices, barycentric, tet_index):
+        self.lib = load_library()
+        self.solver = solver
+        v = _f64(vertices, (-1, 3))
+        self.n = v.shape[0]
+        w = _f64(barycentric, (self.n, 3))
+        idx = np.ascontiguousarray(np.asarray(tet_index, dtype=np.int32).reshape(self.n))
+        h = C.c_void_p()
+        solver._check(self.lib.otfem_surface_create(solver.h, self.n, _dptr(v), _dptr(w),
+                                                    idx.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                    C.byref(h)))
+        self.h = h
+        self.kernel_ms = 0.0
+
+    def update(self):
+        out = np.zeros((self.n, 3))
+        ms = C.c_double()
+        st = self.lib.otfem_surface_update(self.h, _dptr(out), C.byref(ms))
+        if st != OK:
+            msg = self.lib.otfem_surface_last_error(self.h)
+            raise OtfemError(st, msg.decode() if msg else "")
+        self.kernel_ms = ms.value
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.otfem_surface_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Model:
