@@ -24,6 +24,7 @@
 //   fem::update_original_coord     dynamics/fem/fem_update_original_coord.h:28-33
 //   fem::export_K                  dynamics/fem/fem_export_k.h:23-68
 //   t4mesh::generate_blocks        core/containers/t4mesh/util/t4mesh_block_generator.h:32-110
+//                                  (voxel grid: :119-243)
 //
 // Each fem::simulate call marshals the caller's node fields (m_coord,
 // m_velocity, m_f_external, m_fixed) to the device, runs one step there, and
@@ -248,6 +249,57 @@ void generate_blocks(unsigned int const &I, unsigned int const &J, unsigned int 
           mesh.insert(p2, p5, p7, p6);
           mesh.insert(p0, p7, p5, p4);
           mesh.insert(p2, p0, p7, p5);
+        }
+      }
+}
+
+// t4mesh_block_generator.h:119-243: nodes at the set voxels of a grid (in its
+// storage order, i fastest: p = (k J + j) I + i), placed at min + (i dx, j dy,
+// k dz) (grid_idx2coord.h:29-46); each cell contributes those of its five
+// tetrahedra whose four corners are set.  `grid_type` needs I(), J(), K(),
+// dx(), dy(), dz(), min_coord(int) and linear access voxels(p), as the
+// reference's grid::Grid has.  Nodes in no tetrahedron are kept (the
+// reference reports them).
+template <typename t4mesh_type, typename grid_type>
+void generate_blocks(grid_type &voxels, t4mesh_type &mesh) {
+  mesh.clear();
+  const size_t I = voxels.I(), J = voxels.J(), K = voxels.K();
+  std::vector<long long> node_of(I * J * K, -1);
+  for (size_t k = 0; k < K; ++k)
+    for (size_t j = 0; j < J; ++j)
+      for (size_t i = 0; i < I; ++i) {
+        const size_t p = (k * J + j) * I + i;
+        if (!voxels(p)) continue;
+        auto n = mesh.insert();
+        n->m_coord(0) = i * voxels.dx() + voxels.min_coord(0);
+        n->m_coord(1) = j * voxels.dy() + voxels.min_coord(1);
+        n->m_coord(2) = k * voxels.dz() + voxels.min_coord(2);
+        node_of[p] = (long long)n->idx();
+      }
+  if (I < 2 || J < 2 || K < 2) return;
+  auto add = [&](size_t a, size_t b, size_t c, size_t d) {
+    if (voxels(a) && voxels(b) && voxels(c) && voxels(d))
+      mesh.insert((size_t)node_of[a], (size_t)node_of[b], (size_t)node_of[c], (size_t)node_of[d]);
+  };
+  for (size_t i = 0; i + 1 < I; ++i)
+    for (size_t j = 0; j + 1 < J; ++j)
+      for (size_t k = 0; k + 1 < K; ++k) {
+        const size_t p0 = (k * J + j) * I + i, p1 = (k * J + j) * I + (i + 1);
+        const size_t p2 = (k * J + (j + 1)) * I + (i + 1), p3 = (k * J + (j + 1)) * I + i;
+        const size_t p4 = ((k + 1) * J + j) * I + i, p5 = ((k + 1) * J + j) * I + (i + 1);
+        const size_t p6 = ((k + 1) * J + (j + 1)) * I + (i + 1), p7 = ((k + 1) * J + (j + 1)) * I + i;
+        if ((i + j + k) % 2 == 1) {
+          add(p1, p2, p3, p6);
+          add(p3, p6, p7, p4);
+          add(p1, p4, p5, p6);
+          add(p1, p3, p0, p4);
+          add(p1, p6, p3, p4);
+        } else {
+          add(p2, p0, p1, p5);
+          add(p2, p7, p3, p0);
+          add(p2, p5, p6, p7);
+          add(p0, p7, p4, p5);
+          add(p2, p0, p5, p7);
         }
       }
 }

@@ -82,6 +82,56 @@ def generate_blocks(I: int, J: int, K: int, width: float, height: float, depth: 
     return coords, tets
 
 
+def generate_blocks_from_voxels(voxels, min_coord, max_coord):
+    """Tetrahedra from a voxel grid (t4mesh_block_generator.h:119-243).
+
+    ``voxels`` is a boolean array indexed [i, j, k] (shape (I, J, K), each at
+    least 2); the grid spans ``min_coord`` .. ``max_coord``, so grid point
+    (i, j, k) sits at min + (i dx, j dy, k dz) with dx = (max - min)/(I - 1)
+    (grid_idx2coord.h:29-46, grid.h:171-173).  Every set voxel becomes a node,
+    numbered in the grid's storage order (i fastest); each cell contributes
+    those of its five tetrahedra whose four corners are all set.  Nodes that
+    end up in no tetrahedron are kept, as in the reference (which reports
+    them); the solver rejects them unless they are fixed.
+
+    Returns ``(coords[N,3] float64, tets[T,4] int32)``.
+    """
+    v = np.asarray(voxels, dtype=bool)
+    if v.ndim != 3 or min(v.shape) < 2:
+        raise ValueError("voxels must be a 3-D array with at least 2 points per axis")
+    I, J, K = v.shape
+    lo = np.asarray(min_coord, dtype=np.float64).reshape(3)
+    hi = np.asarray(max_coord, dtype=np.float64).reshape(3)
+    d = (hi - lo) / (np.array([I, J, K]) - 1)
+    flat = v.transpose(2, 1, 0).ravel()          # storage order p = (k J + j) I + i
+    node_of = np.full(flat.size, -1, dtype=np.int64)
+    node_of[flat] = np.arange(int(flat.sum()))
+    kk, jj, ii = np.meshgrid(np.arange(K), np.arange(J), np.arange(I), indexing="ij")
+    sel = flat
+    coords = np.stack([ii.ravel()[sel] * d[0] + lo[0], jj.ravel()[sel] * d[1] + lo[1],
+                       kk.ravel()[sel] * d[2] + lo[2]], axis=1)
+    # cells (i, j, k) in the reference's loop order: i outer, then j, then k
+    ci, cj, ck = np.meshgrid(np.arange(I - 1), np.arange(J - 1), np.arange(K - 1), indexing="ij")
+    ci, cj, ck = ci.ravel(), cj.ravel(), ck.ravel()
+    p0 = (ck * J + cj) * I + ci
+    p1 = (ck * J + cj) * I + ci + 1
+    p2 = (ck * J + cj + 1) * I + ci + 1
+    p3 = (ck * J + cj + 1) * I + ci
+    p4 = ((ck + 1) * J + cj) * I + ci
+    p5 = ((ck + 1) * J + cj) * I + ci + 1
+    p6 = ((ck + 1) * J + cj + 1) * I + ci + 1
+    p7 = ((ck + 1) * J + cj + 1) * I + ci
+    odd = ((ci + cj + ck) % 2) == 1
+    odd_t = np.stack([np.stack(q, 1) for q in ((p1, p2, p3, p6), (p3, p6, p7, p4), (p1, p4, p5, p6),
+                                                (p1, p3, p0, p4), (p1, p6, p3, p4))], 1)
+    even_t = np.stack([np.stack(q, 1) for q in ((p2, p0, p1, p5), (p2, p7, p3, p0), (p2, p5, p6, p7),
+                                                 (p0, p7, p4, p5), (p2, p0, p5, p7))], 1)
+    cand = np.where(odd[:, None, None], odd_t, even_t).reshape(-1, 4)
+    keep = flat[cand].all(axis=1)
+    tets = node_of[cand[keep]].astype(np.int32)
+    return coords, tets
+
+
 def tet_volumes(coords: np.ndarray, tets: np.ndarray) -> np.ndarray:
     """Signed volumes e10.(e20 x e30)/6 (fem_compute_volume.h:27-39)."""
     p = coords[tets]

@@ -148,3 +148,67 @@ def test_tetgen_errors(tmp_path):
         f.write("1 4 0\n1 1 2 3 9\n")
     with pytest.raises(ValueError):
         mesh_io.tetgen_read(base)
+
+
+# ---- voxel block generator (t4mesh_block_generator.h:119-243) ----
+
+VOXEL = os.path.join(HERE, "cpp", "build", "voxel_check")
+
+
+def _faces(tets):
+    f = np.sort(np.concatenate([tets[:, [0, 1, 2]], tets[:, [0, 1, 3]], tets[:, [0, 2, 3]],
+                                tets[:, [1, 2, 3]]]), axis=1)
+    _, counts = np.unique(f, axis=0, return_counts=True)
+    return counts
+
+
+def test_voxel_generator_full_grid_is_a_conforming_box():
+    I, J, K = 4, 3, 5
+    lo, hi = np.array([-1.0, 0.5, 2.0]), np.array([1.0, 2.0, 4.0])
+    c, t = mesh.generate_blocks_from_voxels(np.ones((I, J, K), bool), lo, hi)
+    assert c.shape == (I * J * K, 3) and t.shape == (5 * (I - 1) * (J - 1) * (K - 1), 4)
+    vol = mesh.tet_volumes(c, t)
+    assert (vol > 0).all()
+    assert vol.sum() == pytest.approx(np.prod(hi - lo), rel=1e-13)
+    # corner tets are 1/6 of a cell, the middle one 1/3
+    cell = np.prod((hi - lo) / (np.array([I, J, K]) - 1))
+    assert np.allclose(np.sort(np.unique(np.round(vol / cell, 12))), [1 / 6, 1 / 3])
+    # conforming: interior faces shared by two tets, boundary triangles = 2 per cell face
+    counts = _faces(t)
+    assert set(counts.tolist()) <= {1, 2}
+    nb = 2 * 2 * ((I - 1) * (J - 1) + (J - 1) * (K - 1) + (I - 1) * (K - 1))
+    assert (counts == 1).sum() == nb
+    # grid points in storage order (i fastest)
+    assert np.allclose(c[1] - c[0], [(hi - lo)[0] / (I - 1), 0, 0])
+    assert np.allclose(c[I] - c[0], [0, (hi - lo)[1] / (J - 1), 0])
+
+
+def test_voxel_generator_subset_and_cpp_agree(tmp_path):
+    rng = np.random.default_rng(4)
+    I, J, K = 6, 5, 4
+    vox = rng.random((I, J, K)) < 0.8
+    lo, hi = np.array([0.0, 0.0, 0.0]), np.array([0.5, 0.4, 0.3])
+    c, t = mesh.generate_blocks_from_voxels(vox, lo, hi)
+    cf, tf = mesh.generate_blocks_from_voxels(np.ones_like(vox), lo, hi)
+    # every tet joins set voxels and is one of the full grid's tets
+    assert c.shape[0] == vox.sum()
+    full = {tuple(np.round(cf[q].ravel(), 12)) for q in tf}
+    assert all(tuple(np.round(c[q].ravel(), 12)) in full for q in t)
+    # a tet of the full grid is kept exactly when its four corners are set
+    setpts = {tuple(np.round(p, 12)) for p in c}
+    expect = sum(all(tuple(np.round(cf[n], 12)) in setpts for n in q) for q in tf)
+    assert len(t) == expect
+    assert (mesh.tet_volumes(c, t) > 0).all()
+    # the C++ template produces the same mesh
+    assert os.path.exists(VOXEL), "build() compiles tests/cpp/voxel_check.cpp"
+    g = str(tmp_path / "g.txt")
+    with open(g, "w") as f:
+        f.write(f"{I} {J} {K}\n{lo[0]} {lo[1]} {lo[2]}\n{hi[0]} {hi[1]} {hi[2]}\n")
+        f.write(" ".join(str(int(x)) for x in vox.transpose(2, 1, 0).ravel()) + "\n")
+    r = subprocess.run([VOXEL, g], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0
+    lines = r.stdout.split("\n")
+    n, nt = (int(x) for x in lines[0].split())
+    cc = np.array([[float(x) for x in l.split()] for l in lines[1:1 + n]])
+    tt = np.array([[int(x) for x in l.split()] for l in lines[1 + n:1 + n + nt]])
+    assert np.array_equal(cc, c) and np.array_equal(tt, t)
