@@ -271,6 +271,25 @@ otfem_status otfem_set_warmstart_model(otfem_handle h, otfem_model_handle m);
  * store fails the step). */
 otfem_status otfem_set_recording(otfem_handle h, int on);
 
+/* ---- Checkpoint / resume (no reference counterpart: the reference saves
+ * geometry only) ---- */
+
+/* Writes the solver's simulation state to `path` (binary): rest and current
+ * positions, velocities, external forces, fixed flags, per-tet materials and
+ * plastic parameters, plastic strain, the learned warm start's velocity
+ * history (when a model is attached) and the per-body iteration counters.
+ * Needs otfem_init first.  Pending steps are completed first. */
+otfem_status otfem_save_state(otfem_handle h, const char *path);
+/* Restores such a file into a solver created for the same mesh (node, tet and
+ * body counts, connectivity and body offsets are checked; a mismatch is
+ * OTFEM_INVALID_ARG): it re-initialises the element data from the saved rest
+ * shape and materials and restores every array bit for bit.  The velocity
+ * history is restored when a model is attached to this solver. */
+otfem_status otfem_load_state(otfem_handle h, const char *path);
+/* The network's FP32 parameters, Adam moments and step count. */
+otfem_status otfem_model_save(otfem_model_handle m, const char *path);
+otfem_status otfem_model_load(otfem_model_handle m, const char *path);
+
 /* ---- Surface coupling (skinning a render surface onto the tetrahedra) ---- */
 
 /* t4mesh::mesh_coupling::bind_surface (t4mesh_mesh_coupling.h:100-200): for

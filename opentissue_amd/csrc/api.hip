@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <limits>
 #include <map>
 #include <tuple>
@@ -33,6 +34,7 @@ struct otfem_solver {
   std::vector<int32_t> tets;
   std::vector<uint8_t> fixed_host;
   std::vector<double> young, poisson, density;  // per tet, from the last init
+  double plastic_c[3] = {0.0, 0.0, 0.0};        // yield, creep, max of the last init
   std::vector<double> mass_host, vol_host;
   bool device_ready = false;
   bool initialized = false;
@@ -472,6 +474,9 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
     pp[3 * t + 2] = c_max;
   }
   h->plastic_on = (c_creep > 0.0 && c_max > 0.0);
+  h->plastic_c[0] = c_yield;
+  h->plastic_c[1] = c_creep;
+  h->plastic_c[2] = c_max;
   hipStream_t s = h->stream;
   if (S > 0) {
     HIPCHK(h, hipMemcpyAsync(h->rest64, rest.data(), 9 * S * sizeof(double), hipMemcpyHostToDevice, s));
@@ -1234,6 +1239,185 @@ otfem_status otfem_layout_info(otfem_handle h, int64_t *num_tiles, int64_t *num_
   if (num_slots) *num_slots = (int64_t)h->L.slot_tet.size();
   if (num_halo) *num_halo = (int64_t)h->L.halo.size();
   if (max_tile_nodes) *max_tile_nodes = h->L.max_tile_nodes;
+  return OTFEM_OK;
+}
+
+// ---- checkpoint / resume ----
+//
+// File: "OTFEMSV1", uint32 version 1, uint32 flags (bit 0: velocity history
+// of the learned start present), int64 N, T, B, uint64 connectivity hash,
+// then in the caller's order: x0, x, v, f_ext (double[3N] each), fixed
+// (uint8[N]), E, nu, rho (double[T] each), plastic yield / creep / max
+// (double[3]), plastic strain (double[6T]), with bit 0 the two history
+// velocities (float[4N] each), and the per-body iteration and unconverged
+// step counters (uint64[B] each).
+
+namespace {
+
+uint64_t connectivity_hash(const otfem_solver *h) {
+  uint64_t x = 1469598103934665603ull;
+  auto mix = [&](const void *p, size_t n) {
+    const unsigned char *b = static_cast<const unsigned char *>(p);
+    for (size_t i = 0; i < n; ++i) {
+      x ^= b[i];
+      x *= 1099511628211ull;
+    }
+  };
+  const int64_t dims[3] = {h->L.N, h->L.T, h->L.B};
+  mix(dims, sizeof(dims));
+  mix(h->tets.data(), h->tets.size() * sizeof(int32_t));
+  mix(h->L.body_node_begin.data(), h->L.body_node_begin.size() * sizeof(int64_t));
+  mix(h->L.body_tet_begin.data(), h->L.body_tet_begin.size() * sizeof(int64_t));
+  return x;
+}
+
+// float4 per node: internal order <-> caller order
+void hist_to_caller(const Layout &L, const std::vector<float> &in, std::vector<float> &out) {
+  out.resize(in.size());
+  for (int64_t g = 0; g < L.N; ++g)
+    for (int c = 0; c < 4; ++c) out[4 * L.node_of_internal[g] + c] = in[4 * g + c];
+}
+void hist_to_internal(const Layout &L, const std::vector<float> &in, std::vector<float> &out) {
+  out.resize(in.size());
+  for (int64_t g = 0; g < L.N; ++g)
+    for (int c = 0; c < 4; ++c) out[4 * g + c] = in[4 * L.node_of_internal[g] + c];
+}
+
+}  // namespace
+
+otfem_status otfem_save_state(otfem_handle h, const char *path) {
+  if (!h || !path) return OTFEM_INVALID_ARG;
+  if (!h->initialized) return fail(h, OTFEM_STATE, "call otfem_init first");
+  otfem_status st = ensure_device(h);
+  if (st) return st;
+  const int64_t N = h->L.N, T = h->L.T, B = h->L.B;
+  std::vector<double> x(3 * N), v(3 * N), f(3 * N), tmp(3 * N), pl(6 * T);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (auto [dst, src] : {std::pair<double *, double *>{x.data(), h->x}, {v.data(), h->v},
+                          {f.data(), h->fext}}) {
+    HIPCHK(h, hipMemcpy(tmp.data(), src, 3 * N * sizeof(double), hipMemcpyDeviceToHost));
+    to_caller3(h->L, tmp, dst);
+  }
+  if (T > 0) HIPCHK(h, hipMemcpy(pl.data(), h->plastic, 6 * T * sizeof(double), hipMemcpyDeviceToHost));
+  const bool hist = h->hist1 != nullptr;
+  std::vector<float> h1, h2, raw(4 * N);
+  if (hist) {
+    HIPCHK(h, hipMemcpy(raw.data(), h->hist1, N * sizeof(float4), hipMemcpyDeviceToHost));
+    hist_to_caller(h->L, raw, h1);
+    HIPCHK(h, hipMemcpy(raw.data(), h->hist2, N * sizeof(float4), hipMemcpyDeviceToHost));
+    hist_to_caller(h->L, raw, h2);
+  }
+  std::vector<unsigned long long> it(B), un(B);
+  HIPCHK(h, hipMemcpy(it.data(), h->iter_acc, B * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(un.data(), h->unconv_acc, B * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  std::ofstream o(path, std::ios::binary);
+  if (!o) return fail(h, OTFEM_INVALID_ARG, std::string("cannot write ") + path);
+  auto put = [&](const void *p, size_t n) { o.write(static_cast<const char *>(p), (std::streamsize)n); };
+  const uint32_t version = 1, flags = hist ? 1u : 0u;
+  const int64_t dims[3] = {N, T, B};
+  const uint64_t hash = connectivity_hash(h);
+  put("OTFEMSV1", 8);
+  put(&version, 4);
+  put(&flags, 4);
+  put(dims, sizeof(dims));
+  put(&hash, 8);
+  put(h->x0_host.data(), 3 * N * 8);
+  put(x.data(), 3 * N * 8);
+  put(v.data(), 3 * N * 8);
+  put(f.data(), 3 * N * 8);
+  put(h->fixed_host.data(), N);
+  put(h->young.data(), T * 8);
+  put(h->poisson.data(), T * 8);
+  put(h->density.data(), T * 8);
+  put(h->plastic_c, 3 * 8);
+  put(pl.data(), 6 * T * 8);
+  if (hist) {
+    put(h1.data(), 4 * N * 4);
+    put(h2.data(), 4 * N * 4);
+  }
+  put(it.data(), B * 8);
+  put(un.data(), B * 8);
+  if (!o) return fail(h, OTFEM_INVALID_ARG, std::string("write failed: ") + path);
+  return OTFEM_OK;
+}
+
+otfem_status otfem_load_state(otfem_handle h, const char *path) {
+  if (!h || !path) return OTFEM_INVALID_ARG;
+  std::ifstream in(path, std::ios::binary);
+  if (!in) return fail(h, OTFEM_INVALID_ARG, std::string("cannot read ") + path);
+  auto get = [&](void *p, size_t n) { in.read(static_cast<char *>(p), (std::streamsize)n); };
+  char magic[8];
+  uint32_t version = 0, flags = 0;
+  int64_t dims[3] = {0, 0, 0};
+  uint64_t hash = 0;
+  get(magic, 8);
+  get(&version, 4);
+  get(&flags, 4);
+  get(dims, sizeof(dims));
+  get(&hash, 8);
+  if (!in || std::memcmp(magic, "OTFEMSV1", 8) != 0 || version != 1)
+    return fail(h, OTFEM_INVALID_ARG, std::string("not a solver state file: ") + path);
+  const int64_t N = h->L.N, T = h->L.T, B = h->L.B;
+  if (dims[0] != N || dims[1] != T || dims[2] != B || hash != connectivity_hash(h))
+    return fail(h, OTFEM_INVALID_ARG, "state file is of another mesh (node / tet counts, "
+                                      "connectivity or body offsets differ)");
+  std::vector<double> x0(3 * N), x(3 * N), v(3 * N), f(3 * N), E(T), nu(T), rho(T), pl(6 * T);
+  std::vector<uint8_t> fixed(N);
+  double pc[3];
+  std::vector<float> h1(4 * N), h2(4 * N);
+  std::vector<unsigned long long> it(B), un(B);
+  get(x0.data(), 3 * N * 8);
+  get(x.data(), 3 * N * 8);
+  get(v.data(), 3 * N * 8);
+  get(f.data(), 3 * N * 8);
+  get(fixed.data(), N);
+  get(E.data(), T * 8);
+  get(nu.data(), T * 8);
+  get(rho.data(), T * 8);
+  get(pc, 3 * 8);
+  get(pl.data(), 6 * T * 8);
+  if (flags & 1u) {
+    get(h1.data(), 4 * N * 4);
+    get(h2.data(), 4 * N * 4);
+  }
+  get(it.data(), B * 8);
+  get(un.data(), B * 8);
+  if (!in) return fail(h, OTFEM_INVALID_ARG, std::string("state file is truncated: ") + path);
+  otfem_status st = ensure_device(h);
+  if (st) return st;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  // rest shape and fixed flags first: the element data and masses follow them
+  h->x0_host = x0;
+  for (int64_t i = 0; i < N; ++i) h->fixed_host[i] = fixed[i] ? 1 : 0;
+  std::vector<double> tmp;
+  std::vector<uint8_t> fx(N);
+  for (int64_t g = 0; g < N; ++g) fx[g] = h->fixed_host[h->L.node_of_internal[g]];
+  to_internal3(h->L, x0.data(), tmp);
+  HIPCHK(h, hipMemcpy(h->x0, tmp.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->fixed, fx.data(), N, hipMemcpyHostToDevice));
+  st = init_common(h, E.data(), nu.data(), rho.data(), false, pc[0], pc[1], pc[2]);
+  if (st) return st;
+  if (h->free_rank) {
+    st = ws_upload_free(h);
+    if (st) return st;
+  }
+  for (auto [src, dst] : {std::pair<const double *, double *>{x.data(), h->x}, {v.data(), h->v},
+                          {f.data(), h->fext}}) {
+    to_internal3(h->L, src, tmp);
+    HIPCHK(h, hipMemcpy(dst, tmp.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice));
+  }
+  if (T > 0) HIPCHK(h, hipMemcpy(h->plastic, pl.data(), 6 * T * sizeof(double), hipMemcpyHostToDevice));
+  if ((flags & 1u) && h->hist1) {
+    std::vector<float> raw;
+    hist_to_internal(h->L, h1, raw);
+    HIPCHK(h, hipMemcpy(h->hist1, raw.data(), N * sizeof(float4), hipMemcpyHostToDevice));
+    hist_to_internal(h->L, h2, raw);
+    HIPCHK(h, hipMemcpy(h->hist2, raw.data(), N * sizeof(float4), hipMemcpyHostToDevice));
+  }
+  HIPCHK(h, hipMemcpy(h->iter_acc, it.data(), B * sizeof(unsigned long long), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(h->unconv_acc, un.data(), B * sizeof(unsigned long long), hipMemcpyHostToDevice));
+  h->rotations_valid = false;
+  drop_graphs(h);
   return OTFEM_OK;
 }
 

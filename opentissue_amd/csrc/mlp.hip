@@ -39,6 +39,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <fstream>
 #include <random>
 #include <string>
 #include <vector>
@@ -1199,6 +1200,56 @@ otfem_status otfem_model_get_params(otfem_model_handle m, float *theta) {
   if (!m || !theta) return OTFEM_INVALID_ARG;
   MCHK(m, hipDeviceSynchronize());
   MCHK(m, hipMemcpy(theta, m->theta, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+  return OTFEM_OK;
+}
+
+// File: "OTFEMML1", uint32 version 1, uint32 parameter count, int64 Adam
+// step, then theta, Adam first and second moments (FP32 each).
+otfem_status otfem_model_save(otfem_model_handle m, const char *path) {
+  if (!m || !path) return OTFEM_INVALID_ARG;
+  std::vector<float> th(kMlpParams), am(kMlpParams), av(kMlpParams);
+  MCHK(m, hipDeviceSynchronize());
+  MCHK(m, hipMemcpy(th.data(), m->theta, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+  MCHK(m, hipMemcpy(am.data(), m->adam_m, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+  MCHK(m, hipMemcpy(av.data(), m->adam_v, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+  std::ofstream f(path, std::ios::binary);
+  if (!f) return mfail(m, OTFEM_INVALID_ARG, std::string("cannot write ") + path);
+  const uint32_t version = 1, count = kMlpParams;
+  const int64_t t = m->adam_t;
+  f.write("OTFEMML1", 8);
+  f.write(reinterpret_cast<const char *>(&version), 4);
+  f.write(reinterpret_cast<const char *>(&count), 4);
+  f.write(reinterpret_cast<const char *>(&t), 8);
+  for (auto *v : {&th, &am, &av}) f.write(reinterpret_cast<const char *>(v->data()), kMlpParams * 4);
+  if (!f) return mfail(m, OTFEM_INVALID_ARG, std::string("write failed: ") + path);
+  return OTFEM_OK;
+}
+
+otfem_status otfem_model_load(otfem_model_handle m, const char *path) {
+  if (!m || !path) return OTFEM_INVALID_ARG;
+  otfem_status st = model_device_check(m);
+  if (st) return st;
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return mfail(m, OTFEM_INVALID_ARG, std::string("cannot read ") + path);
+  char magic[8];
+  uint32_t version = 0, count = 0;
+  int64_t t = 0;
+  f.read(magic, 8);
+  f.read(reinterpret_cast<char *>(&version), 4);
+  f.read(reinterpret_cast<char *>(&count), 4);
+  f.read(reinterpret_cast<char *>(&t), 8);
+  if (!f || std::memcmp(magic, "OTFEMML1", 8) != 0 || version != 1 || count != (uint32_t)kMlpParams ||
+      t < 0)
+    return mfail(m, OTFEM_INVALID_ARG, std::string("not a model file of this network: ") + path);
+  std::vector<float> th(kMlpParams), am(kMlpParams), av(kMlpParams);
+  for (auto *v : {&th, &am, &av}) f.read(reinterpret_cast<char *>(v->data()), kMlpParams * 4);
+  if (!f) return mfail(m, OTFEM_INVALID_ARG, std::string("model file is truncated: ") + path);
+  MCHK(m, hipDeviceSynchronize());
+  st = upload_params(m, th);  // also refreshes the BF16 fragments; zeroes Adam
+  if (st) return st;
+  MCHK(m, hipMemcpy(m->adam_m, am.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+  MCHK(m, hipMemcpy(m->adam_v, av.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+  m->adam_t = t;
   return OTFEM_OK;
 }
 

@@ -142,6 +142,10 @@ SIGNATURES = {
                                                 C.POINTER(C.c_int32), C.c_int, _dp, _fp]),
     "otfem_set_warmstart_model": (C.c_int, [_H, _M]),
     "otfem_set_recording": (C.c_int, [_H, C.c_int]),
+    "otfem_save_state": (C.c_int, [_H, C.c_char_p]),
+    "otfem_load_state": (C.c_int, [_H, C.c_char_p]),
+    "otfem_model_save": (C.c_int, [_M, C.c_char_p]),
+    "otfem_model_load": (C.c_int, [_M, C.c_char_p]),
     "otfem_surface_bind": (C.c_int, [_H, C.c_int64, _dp, _dp, C.POINTER(C.c_int32)]),
     "otfem_surface_create": (C.c_int, [_H, C.c_int64, _dp, _dp, C.POINTER(C.c_int32),
                                        C.POINTER(C.c_void_p)]),
@@ -383,6 +387,14 @@ class Solver:
             cols.ctypes.data_as(C.POINTER(C.c_int32)), _dptr(blocks)))
         return rp, cols, blocks
 
+    def save_state(self, path):
+        """Checkpoint of the simulation state (``otfem_save_state``)."""
+        self._check(self.lib.otfem_save_state(self.h, os.fsencode(path)))
+
+    def load_state(self, path):
+        """Restore a checkpoint of this mesh (``otfem_load_state``)."""
+        self._check(self.lib.otfem_load_state(self.h, os.fsencode(path)))
+
     def surface_bind(self, vertices):
         """t4mesh_mesh_coupling.h:100-200: (barycentric [n, 3], tet_index [n])
         of surface vertices [n, 3] at the current node positions; index -1
@@ -486,6 +498,13 @@ class Model:
             msg = self.lib.otfem_model_last_error(self.h)
             raise OtfemError(st, msg.decode() if msg else "")
         return st
+
+    def save(self, path):
+        """Parameters, Adam moments and step count (``otfem_model_save``)."""
+        self._check(self.lib.otfem_model_save(self.h, os.fsencode(path)))
+
+    def load(self, path):
+        self._check(self.lib.otfem_model_load(self.h, os.fsencode(path)))
 
     def params(self):
         th = np.empty(MLP_PARAMS, dtype=np.float32)
