@@ -437,3 +437,16 @@ def test_export_K_structure(oracle_mod):
     Kff = K[np.ix_(fr, fr)]
     np.testing.assert_allclose(Kff, Kff.T, atol=1e-12 * np.abs(Kff).max())
     np.testing.assert_allclose(Kff, o.dense(0)[np.ix_(fr, fr)], rtol=0, atol=0)
+
+
+def test_oracle_clean_under_address_and_undefined_behaviour_sanitizers():
+    """The oracle built with -fsanitize=address,undefined (oracle/Makefile,
+    target sanitize_check) through plasticity, both CG modes, polar
+    rotation, no-warp, the row export and re-initialisation."""
+    import subprocess
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+    subprocess.run(["make", "-s", "-C", here, "sanitize_check"], check=True, timeout=600)
+    r = subprocess.run([os.path.join(here, "build", "sanitize_check")], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-4000:]
+    assert "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr
