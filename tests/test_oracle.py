@@ -443,6 +443,7 @@ def test_oracle_clean_under_address_and_undefined_behaviour_sanitizers():
     """The oracle built with -fsanitize=address,undefined (oracle/Makefile,
     target sanitize_check) through plasticity, both CG modes, polar
     rotation, no-warp, the row export and re-initialisation."""
+    import os
     import subprocess
     here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
     subprocess.run(["make", "-s", "-C", here, "sanitize_check"], check=True, timeout=600)
