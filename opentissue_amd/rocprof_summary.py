@@ -5,8 +5,11 @@
         (ms), share of the traced kernel time, grid (workgroups), block, LDS;
         with --split-us, dispatches shorter than T us are listed in a row of
         their own (CG kernels issued after every body stopped exit at once)
-    python -m opentissue_amd.rocprof_summary pmc RESULTS.db [...]
-        per-kernel counter values averaged over dispatches (JSON lines)
+    python -m opentissue_amd.rocprof_summary pmc RESULTS.db [...] [--drop-below F]
+        per-kernel counter values averaged over dispatches (JSON lines); with
+        --drop-below F, dispatches whose counter value is below F times that
+        kernel's largest are left out of the average (CG launches issued after
+        every body stopped read almost nothing and would dilute it)
 
 Only reads the database; nothing is executed on the device.
 """
@@ -37,20 +40,32 @@ def kernels(db):
     return out
 
 
-def pmc(db):
+def pmc(db, drop_below=None):
+    """{kernel: {counter: mean per dispatch}} (plus "dispatches": the number
+    averaged).  With drop_below, a dispatch counts only if its value of the
+    kernel's first counter is at least drop_below times the largest one."""
     con = sqlite3.connect(db)
     cur = con.cursor()
     names = _names(cur)
     pm = {r[0]: r[1] for r in cur.execute("select id, name from rocpd_info_pmc")}
-    agg = collections.defaultdict(lambda: collections.defaultdict(float))
-    cnt = collections.defaultdict(set)
+    per = collections.defaultdict(lambda: collections.defaultdict(lambda: collections.defaultdict(float)))
     q = ("select d.kernel_id, e.pmc_id, e.value, d.id from rocpd_pmc_event e "
          "join rocpd_kernel_dispatch d on e.event_id = d.event_id")
     for kid, pid, val, did in cur.execute(q):
-        n = str(names.get(kid, kid))
-        agg[n][pm.get(pid)] += val
-        cnt[n].add(did)
-    return {k: {c: v / len(cnt[k]) for c, v in d.items()} for k, d in agg.items()}
+        per[str(names.get(kid, kid))][did][pm.get(pid)] += val
+    out = {}
+    for k, disp in per.items():
+        rows = list(disp.values())
+        if drop_below is not None and rows:
+            first = sorted(rows[0])[0]
+            top = max(r.get(first, 0.0) for r in rows)
+            rows = [r for r in rows if r.get(first, 0.0) >= drop_below * top]
+        if not rows:
+            continue
+        keys = sorted({c for r in rows for c in r})
+        out[k] = {c: sum(r.get(c, 0.0) for r in rows) / len(rows) for c in keys}
+        out[k]["dispatches"] = len(rows)
+    return out
 
 
 def trace_table(db, match=None, split_us=None):
@@ -91,8 +106,14 @@ def main(argv):
             split = float(argv[argv.index("--split-us") + 1])
         print(trace_table(argv[1], match, split))
     else:
-        for db in argv[1:]:
-            for k, d in pmc(db).items():
+        drop = None
+        args = list(argv[1:])
+        if "--drop-below" in args:
+            i = args.index("--drop-below")
+            drop = float(args[i + 1])
+            del args[i:i + 2]
+        for db in args:
+            for k, d in pmc(db, drop).items():
                 print(json.dumps({"db": db, "kernel": k, "counters": d}))
     return 0
 
