@@ -228,6 +228,49 @@ def test_learned_start_parity_with_oracle():
     assert moved >= 5  # the network did change the start
 
 
+def test_learned_start_on_batched_bodies_matches_per_body_oracle():
+    """One network shared by uneven bodies stepped together, each with its own
+    CG: every body stays within the bar of an oracle run of that body alone
+    (free running, per-body materials and tractions)."""
+    from opentissue_amd import fem, mesh
+    import oracle
+    rng = np.random.default_rng(12)
+    dims = [(2, 3, 2), (3, 2, 4), (1, 2, 2), (4, 2, 2)]
+    parts, orcs, forces, fixed, E, nu = [], [], [], [], [], []
+    for b, d in enumerate(dims):
+        c, t = mesh.generate_blocks(*d, 0.01, 0.01, 0.01)
+        c = mesh.jitter_interior(c, t, 0.01, 0.2, 40 + b)
+        fx = mesh.face_nodes(c, 0, 0).astype(np.uint8)
+        Eb, nub = float(np.exp(rng.uniform(np.log(1e5), np.log(1e6)))), float(rng.uniform(0.25, 0.45))
+        f = mesh.traction_forces(c, t, 0, 1, rng.normal(size=3) * 1e3)
+        f[fx == 1] = 0
+        o = oracle.Oracle(c, t, fx)
+        o.init(Eb, nub, 1000.0)
+        o.set_external_forces(f)
+        parts.append((c, t)); orcs.append(o); forces.append(f); fixed.append(fx)
+        E.append(np.full(len(t), Eb)); nu.append(np.full(len(t), nub))
+    coords, tets, no, to = mesh.pack_bodies(parts)
+    s = fem.Solver(coords, tets, np.concatenate(fixed), body_node_offsets=no, body_tet_offsets=to,
+                   target_tile_nodes=16)
+    s.init_materials(np.concatenate(E), np.concatenate(nu), 1000.0)
+    s.set_external_forces(np.concatenate(forces))
+    model = fem.Model(seed=9)
+    model.set_params(_theta(rng, out_scale=0.2))
+    s.set_warmstart_model(model)
+    L = fem.config_params()
+    L.warm_start = fem.WARM_LEARNED
+    worst = 0.0
+    for _ in range(6):
+        st = s.simulate(0.01, True, L)
+        assert st.flags == 0 and st.bodies_converged == len(dims)
+        x, _ = s.state()
+        for b, o in enumerate(orcs):
+            o.simulate(0.01, True, oracle.config_params())
+            sl = slice(no[b], no[b + 1])
+            worst = max(worst, rel_disp_error(x[sl], o.state()[0], parts[b][0]))
+    assert worst <= 1e-6, worst
+
+
 def test_nonfinite_prediction_falls_back_to_plain():
     from opentissue_amd import fem
     import oracle.mlp as M
