@@ -2,8 +2,10 @@
 #include "layout.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <numeric>
+#include <thread>
 #include <unordered_map>
 
 namespace otfem {
@@ -30,7 +32,6 @@ void rcb(std::vector<int32_t> &idx, int64_t lo, int64_t hi, int64_t parts, const
   for (int c = 1; c < 3; ++c)
     if (mx[c] - mn[c] > mx[ax] - mn[ax]) ax = c;
   int64_t left_parts = parts / 2;
-  int64_t n = hi - lo;
   std::sort(idx.begin() + lo, idx.begin() + hi, [&](int32_t a, int32_t b) {
     double va = x[3 * (int64_t)a + ax], vb = x[3 * (int64_t)b + ax];
     if (va != vb) return va < vb;
@@ -47,14 +48,87 @@ void rcb(std::vector<int32_t> &idx, int64_t lo, int64_t hi, int64_t parts, const
   rcb(idx, mid, hi, parts - left_parts, x, w, part_sizes);
 }
 
+// Runs f(i) for i in [0, n) on up to `threads` threads (the caller's thread
+// is one of them); items are handed out one at a time, so uneven bodies and
+// tiles balance.  Each f(i) must write only to data that item owns.
+template <class F>
+void parallel_for(int64_t n, int threads, F &&f) {
+  const int64_t nt = std::min<int64_t>(threads, n);
+  if (nt <= 1) {
+    for (int64_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<int64_t> next{0};
+  auto work = [&] {
+    for (int64_t i; (i = next.fetch_add(1, std::memory_order_relaxed)) < n;) f(i);
+  };
+  std::vector<std::thread> pool;
+  pool.reserve(nt - 1);
+  for (int64_t t = 1; t < nt; ++t) pool.emplace_back(work);
+  work();
+  for (auto &t : pool) t.join();
+}
+
+// Orders the slots of one tile in windows of 64 (one wave instruction) in
+// which no two slots share an owned corner, so the per-corner LDS atomics of a
+// wave never hit the same address.  Windows are filled in turn with the first
+// remaining compatible slots; a window that cannot be completed is topped up
+// with the next remaining slots so later windows stay aligned.
+void order_slots(const TileInfo &ti, uint16_t *slot_conn, int32_t *slot_tet) {
+  const int32_t n = ti.n_slots;
+  const uint16_t *sc = slot_conn + 4 * (int64_t)ti.slot_begin;
+  std::vector<int32_t> order;
+  order.reserve(n);
+  std::vector<char> taken(n, 0);
+  std::vector<int32_t> mark(ti.n_owned, -1);
+  int32_t first = 0, window = 0;
+  while ((int32_t)order.size() < n) {
+    int32_t got = 0;
+    for (int32_t k = first; k < n && got < 64; ++k) {
+      if (taken[k]) continue;
+      bool ok = true;
+      for (int c = 0; c < 4 && ok; ++c) {
+        int32_t li = sc[4 * k + c];
+        if (li < ti.n_owned && mark[li] == window) ok = false;
+      }
+      if (!ok) continue;
+      for (int c = 0; c < 4; ++c) {
+        int32_t li = sc[4 * k + c];
+        if (li < ti.n_owned) mark[li] = window;
+      }
+      taken[k] = 1;
+      order.push_back(k);
+      ++got;
+    }
+    for (int32_t k = first; k < n && got < 64; ++k) {
+      if (taken[k]) continue;
+      taken[k] = 1;
+      order.push_back(k);
+      ++got;
+    }
+    while (first < n && taken[first]) ++first;
+    ++window;
+  }
+  std::vector<uint16_t> conn(4 * (size_t)n);
+  std::vector<int32_t> tet(n);
+  int32_t *st = slot_tet + ti.slot_begin;
+  for (int32_t k = 0; k < n; ++k) {
+    for (int c = 0; c < 4; ++c) conn[4 * k + c] = sc[4 * order[k] + c];
+    tet[k] = st[order[k]];
+  }
+  std::copy(conn.begin(), conn.end(), slot_conn + 4 * (int64_t)ti.slot_begin);
+  std::copy(tet.begin(), tet.end(), st);
+}
+
 }  // namespace
 
 std::string build_layout(int64_t N, int64_t T, const double *coords, const int32_t *tets,
                          int64_t B, const int64_t *body_node_off, const int64_t *body_tet_off,
-                         int target_tile_nodes, int max_tile_nodes, Layout &L) {
+                         int target_tile_nodes, int max_tile_nodes, Layout &L, int threads) {
   L = Layout();
   if (N <= 0 || T < 0) return "num_nodes must be positive and num_tets non-negative";
   if (N >= (int64_t)INT32_MAX || T >= (int64_t)INT32_MAX) return "mesh too large for 32-bit ids";
+  if (threads <= 0) threads = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
   // Default tiling: about 512 owned nodes per tile; a body large enough to
   // fill the GPU gets a tile count that is a multiple of the 256 compute
   // units of an MI355X, so every CU runs the same number of tiles.
@@ -79,20 +153,30 @@ std::string build_layout(int64_t N, int64_t T, const double *coords, const int32
           L.body_tet_begin[b + 1] < L.body_tet_begin[b])
         return "body offsets must be increasing (every body needs a node)";
   }
-  for (int64_t t = 0; t < T; ++t)
-    for (int k = 0; k < 4; ++k) {
-      int64_t v = tets[4 * t + k];
-      if (v < 0 || v >= N) return "tetrahedron node index out of range";
-    }
-  // tets must stay inside their body
-  if (L.B > 1) {
-    for (int64_t b = 0; b < L.B; ++b)
-      for (int64_t t = L.body_tet_begin[b]; t < L.body_tet_begin[b + 1]; ++t)
+  const int64_t nb = L.B;
+  const int64_t *bn = L.body_node_begin.data();
+  const int64_t *bt = L.body_tet_begin.data();
+  // Every tet must index a node of its own body (the whole mesh when there
+  // is one body).  After this check the bodies share no node and no tet, so
+  // the per-body passes below write disjoint data.
+  {
+    std::atomic<int> bad{0};
+    parallel_for(nb, threads, [&](int64_t b) {
+      for (int64_t t = bt[b]; t < bt[b + 1]; ++t)
         for (int k = 0; k < 4; ++k) {
           int64_t v = tets[4 * t + k];
-          if (v < L.body_node_begin[b] || v >= L.body_node_begin[b + 1])
-            return "a tetrahedron references a node of another body";
+          if (v < 0 || v >= N) {
+            bad.store(1);
+            return;
+          }
+          if (v < bn[b] || v >= bn[b + 1]) {  // reported only if no index is out of range
+            int expected = 0;
+            bad.compare_exchange_strong(expected, 2);
+          }
         }
+    });
+    if (bad.load() == 1) return "tetrahedron node index out of range";
+    if (bad.load() == 2) return "a tetrahedron references a node of another body";
   }
 
   // 1. Partition nodes of each body into tiles (internal ids = tile-major).
@@ -102,93 +186,101 @@ std::string build_layout(int64_t N, int64_t T, const double *coords, const int32
   // each node by (tets at the node)^3 + 1 in the bisection brings the
   // largest tile to ~2-3% above the mean slot count (64^3 cube: 3,523 vs
   // 3,805 slots at a mean of 3,450).
-  std::vector<double> weight(N, 0.0);
-  for (int64_t q = 0; q < 4 * T; ++q) weight[tets[q]] += 1.0;
-  for (auto &wv : weight) wv = wv * wv * wv + 1.0;
   // The bisection runs on smoothed coordinates (each pass moves a node to the
   // mean centroid of its tets): on a jittered lattice, cuts through the raw
   // coordinates are ragged and cut more tets (64^3 cube, jitter 0.2 h: 1.43
   // slots per tet; after 8 passes 1.37, against 1.35 without jitter).  Nodes
   // of no tet keep their coordinates.
+  std::vector<double> weight(N, 0.0);
   std::vector<double> xs(coords, coords + 3 * N);
+  std::vector<int32_t> order(N);
+  std::vector<std::vector<int32_t>> body_sizes(nb);
   {
     std::vector<double> acc(3 * N);
     std::vector<int32_t> cnt(N);
-    for (int pass = 0; pass < 8 && T > 0; ++pass) {
-      std::fill(acc.begin(), acc.end(), 0.0);
-      std::fill(cnt.begin(), cnt.end(), 0);
-      for (int64_t t = 0; t < T; ++t) {
-        const int32_t *tt = tets + 4 * t;
-        double c[3] = {0.0, 0.0, 0.0};
-        for (int k = 0; k < 4; ++k)
-          for (int d = 0; d < 3; ++d) c[d] += xs[3 * (int64_t)tt[k] + d];
-        for (int k = 0; k < 4; ++k) {
-          for (int d = 0; d < 3; ++d) acc[3 * (int64_t)tt[k] + d] += 0.25 * c[d];
-          ++cnt[tt[k]];
+    parallel_for(nb, threads, [&](int64_t b) {
+      const int64_t n0 = bn[b], n1 = bn[b + 1];
+      for (int64_t q = 4 * bt[b]; q < 4 * bt[b + 1]; ++q) weight[tets[q]] += 1.0;
+      for (int64_t i = n0; i < n1; ++i) weight[i] = weight[i] * weight[i] * weight[i] + 1.0;
+      for (int pass = 0; pass < 8 && bt[b + 1] > bt[b]; ++pass) {
+        std::fill(acc.begin() + 3 * n0, acc.begin() + 3 * n1, 0.0);
+        std::fill(cnt.begin() + n0, cnt.begin() + n1, 0);
+        for (int64_t t = bt[b]; t < bt[b + 1]; ++t) {
+          const int32_t *tt = tets + 4 * t;
+          double c[3] = {0.0, 0.0, 0.0};
+          for (int k = 0; k < 4; ++k)
+            for (int d = 0; d < 3; ++d) c[d] += xs[3 * (int64_t)tt[k] + d];
+          for (int k = 0; k < 4; ++k) {
+            for (int d = 0; d < 3; ++d) acc[3 * (int64_t)tt[k] + d] += 0.25 * c[d];
+            ++cnt[tt[k]];
+          }
         }
+        for (int64_t i = n0; i < n1; ++i)
+          if (cnt[i] > 0)
+            for (int d = 0; d < 3; ++d) xs[3 * i + d] = acc[3 * i + d] / cnt[i];
       }
-      for (int64_t i = 0; i < N; ++i)
-        if (cnt[i] > 0)
-          for (int d = 0; d < 3; ++d) xs[3 * i + d] = acc[3 * i + d] / cnt[i];
-    }
+      // bisection of the body's nodes; the body's internal ids are
+      // [n0, n1) because bodies keep their order
+      std::iota(order.begin() + n0, order.begin() + n1, (int32_t)n0);
+      int64_t parts = std::max<int64_t>(1, (n1 - n0 + target_tile_nodes - 1) / target_tile_nodes);
+      if (auto_tiles && nb == 1) {
+        const int64_t cus = 256;
+        const int64_t k = (int64_t)std::llround((double)(n1 - n0) / (512.0 * cus));
+        if (k >= 1) parts = k * cus;
+      }
+      rcb(order, n0, n1, parts, xs.data(), weight.data(), body_sizes[b]);
+      // keep user order inside a tile (locality of generator order)
+      int64_t pos = n0;
+      for (int32_t s : body_sizes[b]) {
+        std::sort(order.begin() + pos, order.begin() + pos + s);
+        pos += s;
+      }
+    });
   }
-  L.node_of_internal.resize(N);
+  L.body_tile_begin.assign(nb + 1, 0);
+  for (int64_t b = 0; b < nb; ++b)
+    L.body_tile_begin[b + 1] = L.body_tile_begin[b] + (int32_t)body_sizes[b].size();
+  const int64_t ntiles = L.body_tile_begin[nb];
+  L.tiles.assign(ntiles, TileInfo{});
+  L.node_of_internal = std::move(order);
   L.internal_of_node.assign(N, -1);
   std::vector<int32_t> tile_of_internal(N);
-  L.body_tile_begin.assign(L.B + 1, 0);
-  int64_t next_internal = 0;
-  for (int64_t b = 0; b < L.B; ++b) {
-    int64_t n0 = L.body_node_begin[b], n1 = L.body_node_begin[b + 1];
-    std::vector<int32_t> idx(n1 - n0);
-    std::iota(idx.begin(), idx.end(), (int32_t)n0);
-    int64_t parts = std::max<int64_t>(1, (n1 - n0 + target_tile_nodes - 1) / target_tile_nodes);
-    if (auto_tiles && L.B == 1) {
-      const int64_t cus = 256;
-      const int64_t k = (int64_t)std::llround((double)(n1 - n0) / (512.0 * cus));
-      if (k >= 1) parts = k * cus;
-    }
-    std::vector<int32_t> sizes;
-    rcb(idx, 0, (int64_t)idx.size(), parts, xs.data(), weight.data(), sizes);
-    int64_t pos = 0;
-    L.body_tile_begin[b] = (int32_t)L.tiles.size();
-    int32_t rank = 0;
-    for (int32_t s : sizes) {
-      TileInfo ti{};
+  parallel_for(nb, threads, [&](int64_t b) {
+    int64_t next_internal = bn[b];
+    int32_t tile = L.body_tile_begin[b], rank = 0;
+    for (int32_t s : body_sizes[b]) {
+      TileInfo &ti = L.tiles[tile];
       ti.node_begin = (int32_t)next_internal;
       ti.n_owned = s;
       ti.body = (int32_t)b;
       ti.body_rank = rank++;
-      // keep user order inside a tile (locality of generator order)
-      std::sort(idx.begin() + pos, idx.begin() + pos + s);
+      ti.body_tile_begin = L.body_tile_begin[b];
+      ti.body_tile_end = L.body_tile_begin[b + 1];
       for (int32_t k = 0; k < s; ++k) {
-        int32_t u = idx[pos + k];
-        L.node_of_internal[next_internal] = u;
-        L.internal_of_node[u] = (int32_t)next_internal;
-        tile_of_internal[next_internal] = (int32_t)L.tiles.size();
+        L.internal_of_node[L.node_of_internal[next_internal]] = (int32_t)next_internal;
+        tile_of_internal[next_internal] = tile;
         ++next_internal;
       }
-      pos += s;
-      L.tiles.push_back(ti);
+      ++tile;
     }
-  }
-  L.body_tile_begin[L.B] = (int32_t)L.tiles.size();
-  for (auto &ti : L.tiles) {
-    ti.body_tile_begin = L.body_tile_begin[ti.body];
-    ti.body_tile_end = L.body_tile_begin[ti.body + 1];
-  }
-  const int64_t ntiles = (int64_t)L.tiles.size();
+  });
+  body_sizes.clear();
+  body_sizes.shrink_to_fit();
 
   // 2. Slots: each tet goes into every distinct tile owning one of its corners.
+  // A body's tets only reach the body's own tiles.
   std::vector<int64_t> tile_nslots(ntiles, 0);
-  for (int64_t t = 0; t < T; ++t) {
-    int32_t tl[4];
-    for (int k = 0; k < 4; ++k) tl[k] = tile_of_internal[L.internal_of_node[tets[4 * t + k]]];
-    for (int k = 0; k < 4; ++k) {
-      bool dup = false;
-      for (int q = 0; q < k; ++q) dup |= (tl[q] == tl[k]);
-      if (!dup) tile_nslots[tl[k]]++;
+  parallel_for(nb, threads, [&](int64_t b) {
+    for (int64_t t = bt[b]; t < bt[b + 1]; ++t) {
+      int32_t tl[4];
+      for (int k = 0; k < 4; ++k) tl[k] = tile_of_internal[L.internal_of_node[tets[4 * t + k]]];
+      for (int k = 0; k < 4; ++k) {
+        bool dup = false;
+        for (int q = 0; q < k; ++q) dup |= (tl[q] == tl[k]);
+        if (!dup) tile_nslots[tl[k]]++;
+      }
     }
-  }
+  });
   int64_t S = 0;
   for (int64_t i = 0; i < ntiles; ++i) {
     if (S + tile_nslots[i] >= (int64_t)INT32_MAX) return "too many slots for 32-bit offsets";
@@ -199,29 +291,33 @@ std::string build_layout(int64_t N, int64_t T, const double *coords, const int32
   L.slot_conn.assign(4 * S, 0);
   L.slot_tet.assign(S, 0);
   std::vector<int32_t> slot_corner_internal(4 * S);
-  for (int64_t t = 0; t < T; ++t) {
-    int32_t in[4], tl[4];
-    for (int k = 0; k < 4; ++k) {
-      in[k] = L.internal_of_node[tets[4 * t + k]];
-      tl[k] = tile_of_internal[in[k]];
+  parallel_for(nb, threads, [&](int64_t b) {
+    for (int64_t t = bt[b]; t < bt[b + 1]; ++t) {
+      int32_t in[4], tl[4];
+      for (int k = 0; k < 4; ++k) {
+        in[k] = L.internal_of_node[tets[4 * t + k]];
+        tl[k] = tile_of_internal[in[k]];
+      }
+      for (int k = 0; k < 4; ++k) {
+        bool dup = false;
+        for (int q = 0; q < k; ++q) dup |= (tl[q] == tl[k]);
+        if (dup) continue;
+        TileInfo &ti = L.tiles[tl[k]];
+        int64_t s = ti.slot_begin + ti.n_slots++;
+        L.slot_tet[s] = (int32_t)t | (k == 0 ? (int32_t)0x80000000u : 0);
+        for (int c = 0; c < 4; ++c) slot_corner_internal[4 * s + c] = in[c];
+      }
     }
-    for (int k = 0; k < 4; ++k) {
-      bool dup = false;
-      for (int q = 0; q < k; ++q) dup |= (tl[q] == tl[k]);
-      if (dup) continue;
-      TileInfo &ti = L.tiles[tl[k]];
-      int64_t s = ti.slot_begin + ti.n_slots++;
-      L.slot_tet[s] = (int32_t)t | (k == 0 ? (int32_t)0x80000000u : 0);
-      for (int c = 0; c < 4; ++c) slot_corner_internal[4 * s + c] = in[c];
-    }
-  }
+  });
 
-  // 3. Halo tables and tile-local corner indices.
-  std::unordered_map<int32_t, int32_t> local;
-  for (int64_t i = 0; i < ntiles; ++i) {
+  // 3. Halo tables and tile-local corner indices, per tile; the halo lists
+  // are then concatenated in tile order.
+  std::vector<std::vector<int32_t>> tile_halo(ntiles);
+  std::atomic<bool> too_many{false};
+  parallel_for(ntiles, threads, [&](int64_t i) {
     TileInfo &ti = L.tiles[i];
-    ti.halo_begin = (int32_t)L.halo.size();
-    local.clear();
+    std::vector<int32_t> &halo = tile_halo[i];
+    std::unordered_map<int32_t, int32_t> local;
     for (int64_t s = ti.slot_begin; s < ti.slot_begin + ti.n_slots; ++s)
       for (int c = 0; c < 4; ++c) {
         int32_t g = slot_corner_internal[4 * s + c];
@@ -231,65 +327,33 @@ std::string build_layout(int64_t N, int64_t T, const double *coords, const int32
         } else {
           auto it = local.find(g);
           if (it == local.end()) {
-            li = ti.n_owned + (int32_t)(L.halo.size() - ti.halo_begin);
+            li = ti.n_owned + (int32_t)halo.size();
             local.emplace(g, li);
-            L.halo.push_back(g);
+            halo.push_back(g);
           } else {
             li = it->second;
           }
         }
-        if (li > 65535) return "tile node table exceeds 16-bit local indices";
+        if (li > 65535) {
+          too_many.store(true);
+          return;
+        }
         L.slot_conn[4 * s + c] = (uint16_t)li;
       }
-    ti.n_halo = (int32_t)(L.halo.size() - ti.halo_begin);
-    // Order the tile's slots in windows of 64 (one wave instruction) in which
-    // no two slots share an owned corner, so the per-corner LDS atomics of a
-    // wave never hit the same address.  Windows are filled in turn with the
-    // first remaining compatible slots; a window that cannot be completed is
-    // topped up with the next remaining slots so later windows stay aligned.
-    {
-      const int32_t n = ti.n_slots, s0 = ti.slot_begin;
-      std::vector<int32_t> order;
-      order.reserve(n);
-      std::vector<char> taken(n, 0);
-      std::vector<int32_t> mark(ti.n_owned, -1);
-      int32_t first = 0, window = 0;
-      while ((int32_t)order.size() < n) {
-        int32_t got = 0;
-        for (int32_t k = first; k < n && got < 64; ++k) {
-          if (taken[k]) continue;
-          bool ok = true;
-          for (int c = 0; c < 4 && ok; ++c) {
-            int32_t li = L.slot_conn[4 * (s0 + k) + c];
-            if (li < ti.n_owned && mark[li] == window) ok = false;
-          }
-          if (!ok) continue;
-          for (int c = 0; c < 4; ++c) {
-            int32_t li = L.slot_conn[4 * (s0 + k) + c];
-            if (li < ti.n_owned) mark[li] = window;
-          }
-          taken[k] = 1;
-          order.push_back(k);
-          ++got;
-        }
-        for (int32_t k = first; k < n && got < 64; ++k) {
-          if (taken[k]) continue;
-          taken[k] = 1;
-          order.push_back(k);
-          ++got;
-        }
-        while (first < n && taken[first]) ++first;
-        ++window;
-      }
-      std::vector<uint16_t> conn(4 * (size_t)n);
-      std::vector<int32_t> tet(n);
-      for (int32_t k = 0; k < n; ++k) {
-        for (int c = 0; c < 4; ++c) conn[4 * k + c] = L.slot_conn[4 * (s0 + order[k]) + c];
-        tet[k] = L.slot_tet[s0 + order[k]];
-      }
-      std::copy(conn.begin(), conn.end(), L.slot_conn.begin() + 4 * (int64_t)s0);
-      std::copy(tet.begin(), tet.end(), L.slot_tet.begin() + s0);
-    }
+    ti.n_halo = (int32_t)halo.size();
+    order_slots(ti, L.slot_conn.data(), L.slot_tet.data());
+  });
+  if (too_many.load()) return "tile node table exceeds 16-bit local indices";
+  int64_t H = 0;
+  for (int64_t i = 0; i < ntiles; ++i) {
+    L.tiles[i].halo_begin = (int32_t)H;
+    H += L.tiles[i].n_halo;
+  }
+  L.halo.resize(H);
+  parallel_for(ntiles, threads, [&](int64_t i) {
+    std::copy(tile_halo[i].begin(), tile_halo[i].end(), L.halo.begin() + L.tiles[i].halo_begin);
+  });
+  for (const TileInfo &ti : L.tiles) {
     L.max_tile_nodes = std::max(L.max_tile_nodes, ti.n_owned + ti.n_halo);
     L.max_tile_owned = std::max(L.max_tile_owned, ti.n_owned);
     L.max_tile_slots = std::max(L.max_tile_slots, ti.n_slots);

@@ -51,8 +51,13 @@ struct Layout {
 // Builds the layout.  `coords` are rest coordinates [3N] (used only for the
 // spatial partition), `tets` [4T].  Bodies: B == 0/1 means one body spanning
 // everything.  Returns an empty string on success, else an error message.
+// The work runs on `threads` host threads (0: one per hardware thread, at
+// most 16), split by body and by tile; every body and tile is processed in
+// the same order whatever the thread count, so the layout does not depend on
+// it.
 std::string build_layout(int64_t N, int64_t T, const double *coords, const int32_t *tets,
                          int64_t B, const int64_t *body_node_off, const int64_t *body_tet_off,
-                         int target_tile_nodes, int max_tile_nodes, Layout &out);
+                         int target_tile_nodes, int max_tile_nodes, Layout &out,
+                         int threads = 0);
 
 }  // namespace otfem

@@ -101,6 +101,17 @@ def test_batched_layout_never_mixes_bodies(fem):
     s.close()
 
 
+def test_layout_independent_of_thread_count():
+    """tests/cpp/layout_check.cpp: the host layout is the same for 1, 3 and 8
+    threads (batched bodies and one 256-tile body), its slots and halos
+    resolve to the mesh's corners, and error precedence is kept."""
+    exe = os.path.join(ROOT, "tests", "cpp", "build", "layout_check")
+    assert os.path.exists(exe), "build() compiles tests/cpp/layout_check.cpp"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "layout_check ok" in r.stdout
+
+
 def test_calls_before_init_report_state(fem):
     from opentissue_amd import mesh
     c, t = mesh.generate_blocks(1, 1, 1, 0.1, 0.1, 0.1)
