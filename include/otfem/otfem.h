@@ -320,6 +320,57 @@ otfem_status otfem_surface_update(otfem_surface_handle s, double *vertices_out, 
 const char *otfem_surface_last_error(otfem_surface_handle s);
 void otfem_surface_destroy(otfem_surface_handle s);
 
+/* ---- Lattice bodies built on the device (the step before otfem_create) ---- */
+
+/* Random source of the jitter: fills out[3n] with the displacements
+ * (x, y, z) of the n listed nodes, in list order, and returns 0 (anything
+ * else aborts the build).  Called once per round with the nodes to move. */
+typedef int (*otfem_jitter_fn)(void *user, int64_t n, double *out);
+
+typedef struct {
+  int64_t num_bodies;          /* B >= 1 */
+  const int32_t *dims;         /* [3B]: I, J, K cubes of body b (each >= 1) */
+  double width, height, depth; /* cube edges; node (i, j, k) of a body sits at
+                                  (width i, height k, depth j) */
+  /* Jitter (off when jitter == 0): every interior node (not on its body's
+   * bounding box) moves, then the nodes of any tetrahedron whose signed
+   * volume is below min_volume are moved again from their lattice
+   * position, round after round (at most max_rounds). */
+  double jitter;               /* built-in source: U(-jitter e, jitter e) per axis, e that axis's edge */
+  double min_volume;
+  int32_t max_rounds;
+  uint64_t seed;               /* built-in source */
+  otfem_jitter_fn draw;        /* NULL: the built-in source */
+  void *draw_user;
+  /* Boundary conditions on bounding faces of every body: world axis 0 (x),
+   * 1 (y) or 2 (z) and side 0 (minimum) or 1 (maximum); axis -1 for none. */
+  int32_t fixed_axis, fixed_side;
+  int32_t load_axis, load_side;
+  const double *traction;      /* [3B] Pa per body, or [3] for all when traction_per_body == 0 */
+  int32_t traction_per_body;
+} otfem_blocks_desc;
+
+/* Node and tetrahedron counts of the bodies (sum of (I+1)(J+1)(K+1) and
+ * 5 I J K). */
+otfem_status otfem_blocks_count(const otfem_blocks_desc *d, int64_t *num_nodes, int64_t *num_tets);
+
+/* t4mesh::generate_blocks (t4mesh_block_generator.h:32-110) for B bodies at
+ * once, packed as CSR of bodies: node order x, then J, then K; five
+ * tetrahedra per cube with the parity switch of :91-106; tets[4T] hold
+ * global node ids and body b owns nodes [body_node_offsets[b], [b+1]) and
+ * tetrahedra [body_tet_offsets[b], [b+1]).  With a load face, f_ext[3N]
+ * receives the traction as nodal forces t A / 3 per boundary triangle of
+ * that face (the faces of T4BoundaryFaces, t4mesh_t4boundary_faces.h:87-150,
+ * that lie on it), zero at fixed nodes; fixed[N] marks the fixed face.
+ * fixed, f_ext and rounds (jitter rounds taken) may be NULL.  Computed on
+ * the device; the arrays are host memory of the sizes otfem_blocks_count
+ * gives. */
+otfem_status otfem_blocks_build(const otfem_blocks_desc *d, double *coords, int32_t *tets,
+                                uint8_t *fixed, double *f_ext, int64_t *body_node_offsets,
+                                int64_t *body_tet_offsets, int32_t *rounds);
+/* Message of the calling thread's last failed otfem_blocks_* call. */
+const char *otfem_blocks_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -81,6 +81,32 @@ class MeshDesc(C.Structure):
     ]
 
 
+JitterFn = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, C.POINTER(C.c_double))
+
+
+class BlocksDesc(C.Structure):
+    """otfem_blocks_desc (include/otfem/otfem.h)."""
+    _fields_ = [
+        ("num_bodies", C.c_int64),
+        ("dims", C.POINTER(C.c_int32)),
+        ("width", C.c_double),
+        ("height", C.c_double),
+        ("depth", C.c_double),
+        ("jitter", C.c_double),
+        ("min_volume", C.c_double),
+        ("max_rounds", C.c_int32),
+        ("seed", C.c_uint64),
+        ("draw", JitterFn),
+        ("draw_user", C.c_void_p),
+        ("fixed_axis", C.c_int32),
+        ("fixed_side", C.c_int32),
+        ("load_axis", C.c_int32),
+        ("load_side", C.c_int32),
+        ("traction", C.POINTER(C.c_double)),
+        ("traction_per_body", C.c_int32),
+    ]
+
+
 # Every symbol of include/otfem/otfem.h, with (restype, argtypes).
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
@@ -152,6 +178,10 @@ SIGNATURES = {
     "otfem_surface_update": (C.c_int, [C.c_void_p, _dp, _dp]),
     "otfem_surface_last_error": (C.c_char_p, [C.c_void_p]),
     "otfem_surface_destroy": (None, [C.c_void_p]),
+    "otfem_blocks_count": (C.c_int, [C.POINTER(BlocksDesc), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "otfem_blocks_build": (C.c_int, [C.POINTER(BlocksDesc), _dp, C.POINTER(C.c_int32), C.POINTER(C.c_ubyte),
+                                     _dp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    "otfem_blocks_last_error": (C.c_char_p, []),
 }
 
 
@@ -466,6 +496,99 @@ class Surface:
             self.close()
         except Exception:
             pass
+
+
+@dataclass
+class Blocks:
+    """Lattice bodies from :func:`build_blocks` (CSR of bodies)."""
+    coords: np.ndarray              # [N, 3] float64
+    tets: np.ndarray                # [T, 4] int32, global node ids
+    fixed: np.ndarray               # [N] uint8
+    f_ext: np.ndarray               # [N, 3] float64 traction loads
+    body_node_offsets: np.ndarray   # [B+1] int64
+    body_tet_offsets: np.ndarray    # [B+1] int64
+    rounds: int                     # jitter rounds taken
+
+
+_AXIS = {None: -1, "x": 0, "y": 1, "z": 2}
+
+
+def _face(face):
+    """(axis, side) of a face name such as "x-", "x+", "z-"; None: no face."""
+    if face is None:
+        return -1, 0
+    if len(face) != 2 or face[0] not in "xyz" or face[1] not in "+-":
+        raise ValueError("face must be one of x-, x+, y-, y+, z-, z+ or None")
+    return _AXIS[face[0]], 1 if face[1] == "+" else 0
+
+
+def build_blocks(dims, width, height=None, depth=None, jitter=0.0, min_volume=0.0, max_rounds=200,
+                 seed=0, draw=None, fixed_face=None, load_face=None, traction=None) -> Blocks:
+    """Lattice bodies built on the device (``otfem_blocks_build``):
+    ``t4mesh::generate_blocks`` (t4mesh_block_generator.h:32-110) for every
+    row (I, J, K) of ``dims``, packed as CSR of bodies, with optional jitter
+    of interior nodes (volume floor ``min_volume``, redrawn round after
+    round), a fixed face and a traction load face (``"x-"``, ``"x+"``, ...).
+
+    ``draw(n)`` returns the [n, 3] displacements of the n nodes a round
+    moves, in ascending node order; without it the library's own
+    counter-hash source (``seed``, amplitude ``jitter`` times the edge) is
+    used.  ``traction`` is one 3-vector for all bodies or one per body."""
+    lib = load_library()
+    dims = np.ascontiguousarray(np.asarray(dims, dtype=np.int32).reshape(-1, 3))
+    d = BlocksDesc()
+    d.num_bodies = dims.shape[0]
+    d.dims = dims.ctypes.data_as(C.POINTER(C.c_int32))
+    d.width = float(width)
+    d.height = float(width if height is None else height)
+    d.depth = float(width if depth is None else depth)
+    d.jitter = float(jitter)
+    d.min_volume = float(min_volume)
+    d.max_rounds = int(max_rounds)
+    d.seed = int(seed)
+    d.fixed_axis, d.fixed_side = _face(fixed_face)
+    d.load_axis, d.load_side = _face(load_face)
+    tr = None
+    if traction is not None:
+        tr = np.ascontiguousarray(np.asarray(traction, dtype=np.float64))
+        if tr.size not in (3, 3 * dims.shape[0]):
+            raise ValueError("traction must be one 3-vector or one per body")
+        d.traction = _dptr(tr)
+        d.traction_per_body = 1 if tr.size != 3 else 0
+    failure = []
+
+    def _draw(_user, n, out):
+        try:
+            v = np.ascontiguousarray(np.asarray(draw(int(n)), dtype=np.float64).reshape(int(n), 3))
+            C.memmove(out, v.ctypes.data, v.nbytes)
+            return 0
+        except Exception as e:  # reported after the call returns
+            failure.append(e)
+            return 1
+
+    cb = JitterFn(_draw) if draw is not None else JitterFn()
+    d.draw = cb
+    N, T = C.c_int64(), C.c_int64()
+    st = lib.otfem_blocks_count(C.byref(d), C.byref(N), C.byref(T))
+    if st != OK:
+        raise OtfemError(st, lib.otfem_blocks_last_error().decode())
+    N, T, B = N.value, T.value, dims.shape[0]
+    coords = np.empty((N, 3))
+    tets = np.empty((T, 4), dtype=np.int32)
+    fixed = np.empty(N, dtype=np.uint8)
+    f = np.empty((N, 3))
+    no = np.empty(B + 1, dtype=np.int64)
+    to = np.empty(B + 1, dtype=np.int64)
+    rounds = C.c_int32()
+    st = lib.otfem_blocks_build(C.byref(d), _dptr(coords), tets.ctypes.data_as(C.POINTER(C.c_int32)),
+                                fixed.ctypes.data_as(C.POINTER(C.c_ubyte)), _dptr(f),
+                                no.ctypes.data_as(C.POINTER(C.c_int64)),
+                                to.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(rounds))
+    if failure:
+        raise failure[0]
+    if st != OK:
+        raise OtfemError(st, lib.otfem_blocks_last_error().decode())
+    return Blocks(coords, tets, fixed, f, no, to, rounds.value)
 
 
 class Model:

@@ -126,3 +126,32 @@ def test_calls_before_init_report_state(fem):
     if n.value == 0:
         assert s.lib.otfem_init(s.h, 5e5, 0.3, 1000.0, 0, 0, 0) == fem.NO_DEVICE
     s.close()
+
+
+def test_blocks_counts_and_argument_checks(fem):
+    """otfem_blocks_count and the argument checks of otfem_blocks_build run
+    on the host; without a device the build reports NO_DEVICE."""
+    dims = np.array([[3, 4, 5], [1, 1, 1]], dtype=np.int32)
+    d = fem.BlocksDesc()
+    d.num_bodies = 2
+    d.dims = dims.ctypes.data_as(C.POINTER(C.c_int32))
+    d.width = d.height = d.depth = 0.01
+    N, T = C.c_int64(), C.c_int64()
+    assert fem.load_library().otfem_blocks_count(C.byref(d), C.byref(N), C.byref(T)) == fem.OK
+    assert N.value == 4 * 5 * 6 + 8 and T.value == 5 * (60 + 1)
+    with pytest.raises(fem.OtfemError, match="I, J, K >= 1"):
+        fem.build_blocks([(2, 0, 2)], 0.01)
+    with pytest.raises(fem.OtfemError, match="edges must be positive"):
+        fem.build_blocks([(2, 2, 2)], 0.01, height=-1.0)
+    with pytest.raises(ValueError):
+        fem.build_blocks([(2, 2, 2)], 0.01, fixed_face="w+")
+    with pytest.raises(ValueError):
+        fem.build_blocks([(2, 2, 2), (1, 1, 1)], 0.01, load_face="x+", traction=np.ones(4))
+    with pytest.raises(fem.OtfemError, match="jitter must be"):
+        fem.build_blocks([(2, 2, 2)], 0.01, jitter=0.6)
+    n = C.c_int()
+    fem.load_library().otfem_device_count(C.byref(n))
+    if n.value == 0:
+        with pytest.raises(fem.OtfemError) as e:
+            fem.build_blocks([(2, 2, 2)], 0.01)
+        assert e.value.status == fem.NO_DEVICE

@@ -1,0 +1,141 @@
+"""Device lattice builders (otfem_blocks_build) against the host builders of
+opentissue_amd/mesh.py, whose generator, boundary faces, traction loads and
+jitter floor are pinned by known-answer and closed-form tests in
+tests/test_oracle.py (t4mesh_block_generator.h:32-110,
+t4mesh_t4boundary_faces.h:130-150).
+
+Bar: node coordinates, tetrahedra, offsets and fixed masks bit for bit;
+traction loads to 1e-13 of the largest nodal load (the device sums each
+node's triangle contributions in another order than the host's
+np.add.at); the jittered coordinates bit for bit when the device is fed the
+host's random stream.
+"""
+import numpy as np
+import pytest
+
+from opentissue_amd import fem, mesh, workloads
+
+pytestmark = pytest.mark.gpu
+
+H = 0.01
+
+
+def _host_bodies(dims, w, h, d):
+    parts = [mesh.generate_blocks(int(I), int(J), int(K), w, h, d) for I, J, K in dims]
+    return mesh.pack_bodies(parts)
+
+
+def _axis_side(face):
+    return "xyz".index(face[0]), 1 if face[1] == "+" else 0
+
+
+@pytest.mark.parametrize("dims,edges", [
+    ([(1, 1, 1)], (H, H, H)),
+    ([(3, 2, 4), (1, 5, 2), (6, 6, 6)], (0.01, 0.02, 0.03)),
+    ([(4 + b % 5, 2 + (3 * b) % 7, 3 + (5 * b) % 4) for b in range(40)], (H, H, H)),
+])
+def test_lattice_matches_host_generator(dims, edges):
+    print("case dims", dims[:3], "edges", edges, flush=True)
+    w, h, d = edges
+    c, t, no, to = _host_bodies(dims, w, h, d)
+    blk = fem.build_blocks(dims, w, h, d)
+    assert np.array_equal(blk.body_node_offsets, no)
+    assert np.array_equal(blk.body_tet_offsets, to)
+    assert np.array_equal(blk.tets, t)
+    assert np.array_equal(blk.coords, c)
+    assert blk.rounds == 0 and not blk.fixed.any() and not blk.f_ext.any()
+
+
+@pytest.mark.parametrize("fixed_face,load_face", [("x-", "x+"), ("z-", "y+"), ("y-", "z-"), ("x+", "x-")])
+def test_fixed_face_and_traction_match_host(fixed_face, load_face):
+    print("case", fixed_face, load_face, flush=True)
+    dims = [(3, 4, 5), (2, 2, 2), (5, 1, 3)]
+    w, h, d = 0.01, 0.015, 0.02
+    rng = np.random.default_rng(5)
+    tr = rng.normal(size=(len(dims), 3)) * 1e3
+    blk = fem.build_blocks(dims, w, h, d, fixed_face=fixed_face, load_face=load_face, traction=tr)
+    c, t, no, to = _host_bodies(dims, w, h, d)
+    fa, fs = _axis_side(fixed_face)
+    la, ls = _axis_side(load_face)
+    for b in range(len(dims)):
+        cb = c[no[b]:no[b + 1]]
+        tb = t[to[b]:to[b + 1]] - no[b]
+        fx = mesh.face_nodes(cb, fa, fs).astype(np.uint8)
+        assert np.array_equal(blk.fixed[no[b]:no[b + 1]], fx)
+        f = mesh.traction_forces(cb, tb, la, ls, tr[b])
+        f[fx == 1] = 0.0
+        got = blk.f_ext[no[b]:no[b + 1]]
+        assert np.abs(got - f).max() <= 1e-13 * np.abs(f).max()
+        # closed form: with no fixed node on it, the load face carries t * area
+        if not (fx.astype(bool) & mesh.face_nodes(cb, la, ls)).any():
+            ext = np.array([w * dims[b][0], h * dims[b][2], d * dims[b][1]])
+            area = np.prod(np.delete(ext, la))
+            assert np.allclose(got.sum(0), tr[b] * area, rtol=1e-12, atol=0)
+
+
+def test_jitter_with_host_stream_matches_host_jitter():
+    """Resampling rounds included: the same nodes are moved in the same order,
+    so the host's random stream lands on the same nodes."""
+    rng = np.random.default_rng(11)
+    dims = rng.integers(3, 9, size=(48, 3))
+    amp, seed = 0.3 * H, 77
+    c0, t, no, to = _host_bodies(dims, H, H, H)
+    want = mesh.jitter_bodies(c0, t, no, H, 0.3, seed)
+    src = np.random.default_rng(seed)
+    blk = fem.build_blocks(dims, H, jitter=0.3, min_volume=0.05 * H ** 3,
+                           draw=lambda n: src.uniform(-amp, amp, size=(n, 3)))
+    print("rounds", blk.rounds, flush=True)
+    assert blk.rounds >= 2  # the floor made it redraw
+    assert np.array_equal(blk.tets, t)
+    assert np.array_equal(blk.coords, want)
+
+
+def test_single_body_jitter_matches_jitter_interior():
+    amp, seed = 0.35 * H, 3
+    c0, t = mesh.generate_blocks(10, 12, 9, H, H, H)
+    want = mesh.jitter_interior(c0, t, H, 0.35, seed)
+    src = np.random.default_rng(seed)
+    blk = fem.build_blocks([(10, 12, 9)], H, jitter=0.35, min_volume=0.05 * H ** 3,
+                           draw=lambda n: src.uniform(-amp, amp, size=(n, 3)))
+    assert np.array_equal(blk.coords, want)
+
+
+def test_builtin_source_properties():
+    dims = [(6, 5, 7), (3, 3, 3), (8, 2, 4)]
+    kw = dict(jitter=0.3, min_volume=0.05 * H ** 3, seed=1234)
+    a = fem.build_blocks(dims, H, **kw)
+    b = fem.build_blocks(dims, H, **kw)
+    other = fem.build_blocks(dims, H, **dict(kw, seed=1235))
+    c0, t, no, to = _host_bodies(dims, H, H, H)
+    assert np.array_equal(a.coords, b.coords)           # deterministic
+    assert not np.array_equal(a.coords, other.coords)   # seeded
+    disp = a.coords - c0
+    interior = np.zeros(len(c0), bool)
+    for bb in range(len(dims)):
+        cb = c0[no[bb]:no[bb + 1]]
+        lo, hi = cb.min(0), cb.max(0)
+        interior[no[bb]:no[bb + 1]] = np.all((cb > lo + 1e-12) & (cb < hi - 1e-12), axis=1)
+    assert not disp[~interior].any()                    # bounding boxes stay put
+    assert (np.abs(disp[interior]) <= 0.3 * H * (1 + 1e-12)).all()  # x0 + d - x0 may round by an ulp
+    assert np.abs(disp[interior]).max() > 0.2 * H       # it does move them
+    assert (mesh.tet_volumes(a.coords, t) >= 0.05 * H ** 3).all()
+
+
+def test_floor_without_interior_node_is_an_error():
+    # every tet of a single cube has only boundary corners; a floor above the
+    # lattice volumes cannot be met
+    with pytest.raises(fem.OtfemError, match="no interior node"):
+        fem.build_blocks([(1, 1, 1)], H, jitter=0.2, min_volume=H ** 3, seed=1)
+
+
+def test_workloads_same_on_device_and_host():
+    for make in (lambda b: workloads.bodies(count=24, lo=3, hi=10, seed=4, random_traction=True, builder=b),
+                 lambda b: workloads.cube64(n=10, seed=2, builder=b)):
+        dv, hs = make("device"), make("numpy")
+        print("case", dv.name, flush=True)
+        for k in ("coords", "tets", "fixed", "young", "poisson", "density"):
+            assert np.array_equal(getattr(dv, k), getattr(hs, k)), k
+        if dv.body_node_offsets is not None:
+            assert np.array_equal(dv.body_node_offsets, hs.body_node_offsets)
+            assert np.array_equal(dv.body_tet_offsets, hs.body_tet_offsets)
+        assert np.abs(dv.forces - hs.forces).max() <= 1e-13 * np.abs(hs.forces).max()
