@@ -9,6 +9,10 @@ safety cap of 5,000 iterations; `--cap 200` gives the reference case's cap,
 under which most steps of this mesh stop short of 1e-6).
 A step is one full implicit step (rotations + elastic forces + right-hand
 side + CG + position update) over the whole mesh, inputs resident in HBM.
+Setup (not timed): the bodies are built on the device, the layout autotuner
+picks the tile target and CG block size on throw-away solvers
+(`--tile-nodes 0` keeps the default tiling and tunes the block size only),
+then the solver is created.
 
 Output: one JSON line.  `value` is tet-steps/s over exactly --steps timed
 steps from rest with the plain CG start (the reference's v^n); `learned`
@@ -22,7 +26,7 @@ per CG iteration, FP64 v and r, plus 36 B/node for the position update)
 against the MI355X HBM peak; `cpu_baseline` times the FP64 CPU restatement of the reference step
 (one thread) on a bounded sample of the same workload.
 
-    python bench.py [--steps 200] [--warmup 5] [--config cube64|cantilever|bodies]
+    python bench.py [--steps 200] [--warmup 5] [--config cube64|cantilever|bodies|combined]
     python bench.py --copy-probe     # plain device copy of known size, for
                                      # scaling rocprofv3 FETCH/WRITE counters
 """
@@ -188,7 +192,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="time budget of the multi-body CPU sample")
-    ap.add_argument("--tile-nodes", type=int, default=0)
+    ap.add_argument("--tile-nodes", type=int, default=-1,
+                    help="owned nodes per tile (0: the default tiling; -1: the layout autotuner "
+                         "picks the tile target and the CG block size)")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm start legs")
     ap.add_argument("--cap", type=int, default=5000, help="CG iteration cap (safety net)")
@@ -207,17 +213,29 @@ def main():
 
     from opentissue_amd import fem
 
+    t_setup = time.perf_counter()
     w = build_workload(args.config)
-    log(f"workload {w.name}: {w.num_bodies} bodies, {w.num_tets} tets, {w.num_nodes} nodes")
-    s = fem.Solver(w.coords, w.tets, w.fixed, body_node_offsets=w.body_node_offsets,
-                   body_tet_offsets=w.body_tet_offsets, target_tile_nodes=args.tile_nodes)
-    s.init_materials(w.young, w.poisson, w.density)
-    s.set_external_forces(w.forces)
+    t_built = time.perf_counter()
+    log(f"workload {w.name}: {w.num_bodies} bodies, {w.num_tets} tets, {w.num_nodes} nodes "
+        f"(built in {t_built - t_setup:.2f} s)")
     params_kw = dict(alpha=0.1, beta=0.001, tol=1e-6, cap=args.cap)
     P = fem.config_params(**params_kw)
+    tile_nodes, tuned_block = args.tile_nodes, 0
+    if tile_nodes < 0:
+        tile_nodes, tuned_block, _ = fem.autotune_layout(
+            w.coords, w.tets, w.fixed, w.young, w.poisson, w.density, w.forces, w.dt, P, steps=2,
+            body_node_offsets=w.body_node_offsets, body_tet_offsets=w.body_tet_offsets)
+        log(f"layout autotuner: target_tile_nodes {tile_nodes} ({'default tiling' if tile_nodes == 0 else 'owned nodes per tile'}), "
+            f"CG block {tuned_block} ({time.perf_counter() - t_built:.1f} s)")
+    t_create = time.perf_counter()
+    s = fem.Solver(w.coords, w.tets, w.fixed, body_node_offsets=w.body_node_offsets,
+                   body_tet_offsets=w.body_tet_offsets, target_tile_nodes=tile_nodes)
+    log(f"otfem_create (tile layout) {time.perf_counter() - t_create:.2f} s")
+    s.init_materials(w.young, w.poisson, w.density)
+    s.set_external_forces(w.forces)
     layout = s.layout_info()
-    if args.cg_block:
-        best_block = args.cg_block
+    if args.cg_block or tuned_block:
+        best_block = args.cg_block or tuned_block
         s.set_kernel_config(cg_block=best_block)
         log(f"CG block size: {best_block} threads per tile ({layout['tiles']} tiles)")
     else:
@@ -284,6 +302,7 @@ def main():
                         "warm start v^i",
             "tets": T, "nodes": N, "bodies": w.num_bodies,
             "tiles": layout["tiles"], "slots": layout["slots"], "cg_block": best_block,
+            "tile_nodes": tile_nodes, "layout_autotuned": args.tile_nodes < 0,
         },
         "cg_iterations_per_step": total_iters / args.steps / w.num_bodies,
         "cg_body_steps_unconverged": unconverged,
@@ -385,7 +404,7 @@ def main():
         # bounded sample: one step from rest of the same workload; the GPU's
         # first step from the same state is the parity check at full size
         s2 = fem.Solver(w.coords, w.tets, w.fixed, body_node_offsets=w.body_node_offsets,
-                        body_tet_offsets=w.body_tet_offsets, target_tile_nodes=args.tile_nodes)
+                        body_tet_offsets=w.body_tet_offsets, target_tile_nodes=tile_nodes)
         s2.init_materials(w.young, w.poisson, w.density)
         s2.set_external_forces(w.forces)
         s2.simulate(w.dt, True, P)

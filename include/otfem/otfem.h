@@ -168,6 +168,25 @@ otfem_status otfem_set_kernel_config(otfem_handle h, int cg_block, int spmv_mode
  * No reference counterpart (tuning of this implementation). */
 otfem_status otfem_autotune(otfem_handle h, double dt, int use_stiffness_warping,
                             const otfem_params *params, int steps, int *best_block);
+/* Layout autotuner: for each candidate target_tile_nodes it builds a
+ * throw-away solver of `desc` (materials E, nu, rho per tet, loads f_ext or
+ * NULL), runs the otfem_autotune sweep on it, and returns the tile target and
+ * CG block size of the fastest (element + CG device time of `steps` steps;
+ * *best_ms may be NULL).  candidates NULL / num_candidates 0: for one body
+ * the default tiling and 256, 768 and 1,024 tiles, for several bodies 256,
+ * 512 and 1,024 owned nodes per tile.  Candidates that cannot be built (a
+ * tile beyond the LDS budget) are skipped.  Create the real solver with
+ * desc.target_tile_nodes = *best_tile_nodes and apply the block size with
+ * otfem_set_kernel_config.  No reference counterpart (tuning of this
+ * implementation). */
+otfem_status otfem_autotune_layout(const otfem_mesh_desc *desc, const double *young,
+                                   const double *poisson, const double *density,
+                                   const double *f_ext, double dt, int use_stiffness_warping,
+                                   const otfem_params *params, int steps,
+                                   const int32_t *candidates, int num_candidates,
+                                   int32_t *best_tile_nodes, int *best_block, double *best_ms);
+/* Message of the calling thread's last failed otfem_autotune_layout. */
+const char *otfem_autotune_layout_last_error(void);
 /* Per-body CG iterations and final relative residual of the last step. */
 otfem_status otfem_body_stats(otfem_handle h, int *iterations, double *rel_residual);
 

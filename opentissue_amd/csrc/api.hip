@@ -1062,8 +1062,11 @@ otfem_status otfem_set_kernel_config(otfem_handle h, int cg_block, int spmv_mode
   return OTFEM_OK;
 }
 
-otfem_status otfem_autotune(otfem_handle h, double dt, int use_stiffness_warping,
-                            const otfem_params *params, int steps, int *best_block) {
+// Block-size sweep of otfem_autotune; *best_ms is the element + CG device
+// time of the `steps` timed steps at the chosen block size.
+static otfem_status tune_blocks(otfem_handle h, double dt, int use_stiffness_warping,
+                                const otfem_params *params, int steps, int *best_block,
+                                double *best_ms) {
   if (!h) return OTFEM_INVALID_ARG;
   if (!h->initialized) return fail(h, OTFEM_STATE, "call otfem_init before otfem_autotune");
   if (steps <= 0) steps = 2;
@@ -1123,7 +1126,78 @@ otfem_status otfem_autotune(otfem_handle h, double dt, int use_stiffness_warping
   h->lc.cg_block = best_b;
   dfree(sx); dfree(sv); dfree(sf); dfree(spl);
   if (best_block) *best_block = best_b;
+  if (best_ms) *best_ms = best;
   return ret;
+}
+
+otfem_status otfem_autotune(otfem_handle h, double dt, int use_stiffness_warping,
+                            const otfem_params *params, int steps, int *best_block) {
+  return tune_blocks(h, dt, use_stiffness_warping, params, steps, best_block, nullptr);
+}
+
+static thread_local std::string g_tune_err;
+
+const char *otfem_autotune_layout_last_error(void) { return g_tune_err.c_str(); }
+
+otfem_status otfem_autotune_layout(const otfem_mesh_desc *desc, const double *young,
+                                   const double *poisson, const double *density,
+                                   const double *f_ext, double dt, int use_stiffness_warping,
+                                   const otfem_params *params, int steps,
+                                   const int32_t *candidates, int num_candidates,
+                                   int32_t *best_tile_nodes, int *best_block, double *best_ms) {
+  if (!desc || !young || !poisson || !density || !best_tile_nodes || num_candidates < 0 ||
+      (num_candidates > 0 && !candidates) || desc->num_nodes <= 0) {
+    g_tune_err = "invalid argument";
+    return OTFEM_INVALID_ARG;
+  }
+  std::vector<int32_t> cand;
+  if (num_candidates > 0) {
+    cand.assign(candidates, candidates + num_candidates);
+  } else if (desc->num_bodies <= 1) {
+    // one body: the default (a CU-multiple tile count near 512 owned nodes)
+    // and 256, 768 and 1,024 tiles
+    const int64_t N = desc->num_nodes;
+    cand = {0};
+    for (int64_t tiles : {256, 768, 1024})
+      if (N >= 64 * tiles) cand.push_back((int32_t)((N + tiles - 1) / tiles));
+  } else {
+    cand = {256, 512, 1024};
+  }
+  double best = 1e300;
+  int32_t best_c = 0;
+  int best_b = 0;
+  std::string why;
+  for (int32_t c : cand) {
+    otfem_mesh_desc d = *desc;
+    d.target_tile_nodes = c;
+    otfem_handle h = nullptr;
+    otfem_status st = otfem_create(&d, &h);
+    if (st == OTFEM_OK) st = otfem_init_materials(h, young, poisson, density, 0.0, 0.0, 0.0);
+    if (st == OTFEM_OK && f_ext) st = otfem_set_external_forces(h, f_ext);
+    int b = 0;
+    double ms = 1e300;
+    if (st == OTFEM_OK) st = tune_blocks(h, dt, use_stiffness_warping, params, steps, &b, &ms);
+    if (st != OTFEM_OK)
+      why += "target " + std::to_string(c) + ": " + (h ? h->err : std::string("otfem_create failed")) + "; ";
+    if (h) otfem_destroy(h);
+    if (st == OTFEM_NO_DEVICE || st == OTFEM_HIP_ERROR) {
+      g_tune_err = why;
+      return st;
+    }
+    if (st == OTFEM_OK && ms < best) {
+      best = ms;
+      best_c = c;
+      best_b = b;
+    }
+  }
+  if (best == 1e300) {
+    g_tune_err = "no candidate layout could be stepped: " + why;
+    return OTFEM_INVALID_ARG;
+  }
+  *best_tile_nodes = best_c;
+  if (best_block) *best_block = best_b;
+  if (best_ms) *best_ms = best;
+  return OTFEM_OK;
 }
 
 otfem_status otfem_body_stats(otfem_handle h, int *iterations, double *rel_residual) {

@@ -182,6 +182,10 @@ SIGNATURES = {
     "otfem_blocks_build": (C.c_int, [C.POINTER(BlocksDesc), _dp, C.POINTER(C.c_int32), C.POINTER(C.c_ubyte),
                                      _dp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
     "otfem_blocks_last_error": (C.c_char_p, []),
+    "otfem_autotune_layout": (C.c_int, [C.POINTER(MeshDesc), _dp, _dp, _dp, _dp, C.c_double, C.c_int,
+                                        C.POINTER(Params), C.c_int, C.POINTER(C.c_int32), C.c_int,
+                                        C.POINTER(C.c_int32), C.POINTER(C.c_int), _dp]),
+    "otfem_autotune_layout_last_error": (C.c_char_p, []),
 }
 
 
@@ -227,39 +231,74 @@ def _f64(a, shape):
     return np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(shape))
 
 
+def _mesh_desc(coords, tets, fixed=None, velocity=None, model_coords=None, body_node_offsets=None,
+               body_tet_offsets=None, target_tile_nodes=0):
+    """(otfem_mesh_desc, arrays it points into) for a mesh or a CSR-of-bodies
+    batch; keep the arrays alive while the description is used."""
+    coords = _f64(coords, (-1, 3))
+    N = coords.shape[0]
+    tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+    mc = None if model_coords is None else _f64(model_coords, (N, 3))
+    vel = None if velocity is None else _f64(velocity, (N, 3))
+    fx = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.uint8)
+    d = MeshDesc()
+    d.num_nodes, d.num_tets = N, tets.shape[0]
+    d.coord = _dptr(coords)
+    d.model_coord = _dptr(mc) if mc is not None else None
+    d.velocity = _dptr(vel) if vel is not None else None
+    d.fixed = fx.ctypes.data_as(C.POINTER(C.c_ubyte)) if fx is not None else None
+    d.tets = tets.ctypes.data_as(C.POINTER(C.c_int32))
+    bno = bto = None
+    if body_node_offsets is not None:
+        bno = np.ascontiguousarray(body_node_offsets, dtype=np.int64)
+        bto = np.ascontiguousarray(body_tet_offsets, dtype=np.int64)
+        d.num_bodies = len(bno) - 1
+        d.body_node_offsets = bno.ctypes.data_as(C.POINTER(C.c_int64))
+        d.body_tet_offsets = bto.ctypes.data_as(C.POINTER(C.c_int64))
+    d.target_tile_nodes = int(target_tile_nodes)
+    return d, (coords, tets, mc, fx, vel, bno, bto)
+
+
+def autotune_layout(coords, tets, fixed, young, poisson, density, forces, dt, params=None, warp=True,
+                    steps=2, candidates=None, body_node_offsets=None, body_tet_offsets=None):
+    """``otfem_autotune_layout``: (target_tile_nodes, cg_block, ms) of the
+    fastest candidate layout, each timed on a throw-away solver with the
+    block-size sweep of :meth:`Solver.autotune`."""
+    lib = load_library()
+    d, keep = _mesh_desc(coords, tets, fixed, None, None, body_node_offsets, body_tet_offsets)
+    T = d.num_tets
+    E = _f64(np.broadcast_to(np.asarray(young, dtype=np.float64), (T,)), (T,))
+    nu = _f64(np.broadcast_to(np.asarray(poisson, dtype=np.float64), (T,)), (T,))
+    rho = _f64(np.broadcast_to(np.asarray(density, dtype=np.float64), (T,)), (T,))
+    f = None if forces is None else _f64(forces, (d.num_nodes, 3))
+    P = params if params is not None else default_params(0)
+    cand = None if candidates is None else np.ascontiguousarray(candidates, dtype=np.int32)
+    tile, block, ms = C.c_int32(), C.c_int(), C.c_double()
+    st = lib.otfem_autotune_layout(C.byref(d), _dptr(E), _dptr(nu), _dptr(rho),
+                                   _dptr(f) if f is not None else None, float(dt), 1 if warp else 0,
+                                   C.byref(P), int(steps),
+                                   cand.ctypes.data_as(C.POINTER(C.c_int32)) if cand is not None else None,
+                                   0 if cand is None else len(cand), C.byref(tile), C.byref(block),
+                                   C.byref(ms))
+    if st != OK:
+        raise OtfemError(st, lib.otfem_autotune_layout_last_error().decode())
+    return tile.value, block.value, ms.value
+
+
 class Solver:
     """Device solver for one mesh or a CSR-of-bodies batch."""
 
     def __init__(self, coords, tets, fixed=None, velocity=None, model_coords=None,
                  body_node_offsets=None, body_tet_offsets=None, target_tile_nodes=0):
         self.lib = load_library()
-        coords = _f64(coords, (-1, 3))
-        self.N = coords.shape[0]
-        tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
-        self.T = tets.shape[0]
-        mc = None if model_coords is None else _f64(model_coords, (self.N, 3))
-        vel = None if velocity is None else _f64(velocity, (self.N, 3))
-        fx = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.uint8)
+        d, keep = _mesh_desc(coords, tets, fixed, velocity, model_coords, body_node_offsets,
+                             body_tet_offsets, target_tile_nodes)
+        self.N, self.T, self.B = d.num_nodes, d.num_tets, max(1, d.num_bodies)
+        fx = keep[3]
         self.n_free = self.N if fx is None else int((fx == 0).sum())
-        d = MeshDesc()
-        d.num_nodes, d.num_tets = self.N, self.T
-        d.coord = _dptr(coords)
-        d.model_coord = _dptr(mc) if mc is not None else None
-        d.velocity = _dptr(vel) if vel is not None else None
-        d.fixed = fx.ctypes.data_as(C.POINTER(C.c_ubyte)) if fx is not None else None
-        d.tets = tets.ctypes.data_as(C.POINTER(C.c_int32))
-        self.B = 1
-        if body_node_offsets is not None:
-            bno = np.ascontiguousarray(body_node_offsets, dtype=np.int64)
-            bto = np.ascontiguousarray(body_tet_offsets, dtype=np.int64)
-            self.B = len(bno) - 1
-            d.num_bodies = self.B
-            d.body_node_offsets = bno.ctypes.data_as(C.POINTER(C.c_int64))
-            d.body_tet_offsets = bto.ctypes.data_as(C.POINTER(C.c_int64))
-            self._keep_b = (bno, bto)
-        d.target_tile_nodes = int(target_tile_nodes)
         h = C.c_void_p()
-        st = self.lib.otfem_create(C.byref(d), C.byref(h))
+        st = self.lib.otfem_create(C.byref(d), C.byref(h))  # copies what d points to
+        del keep
         if st != OK:
             raise OtfemError(st, "otfem_create failed (invalid mesh description)")
         self.h = h
