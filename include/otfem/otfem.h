@@ -36,7 +36,8 @@ typedef enum {
   OTFEM_NOT_CONVERGED = 4, /* informational: a body hit the iteration cap */
   OTFEM_HIP_ERROR = 5,     /* HIP runtime error or device barrier time-out */
   OTFEM_NO_DEVICE = 6,     /* no usable gfx950 device */
-  OTFEM_STATE = 7          /* call out of order (e.g. step before init) */
+  OTFEM_STATE = 7,         /* call out of order (e.g. step before init) */
+  OTFEM_OUT_OF_MEMORY = 8  /* a host allocation failed */
 } otfem_status;
 
 /* Rotation extraction.  GRAM_SCHMIDT is the reference's

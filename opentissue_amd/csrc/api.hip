@@ -10,6 +10,9 @@
 #include <fstream>
 #include <limits>
 #include <map>
+#include <memory>
+#include <new>
+#include <stdexcept>
 #include <tuple>
 #include <string>
 #include <vector>
@@ -105,6 +108,29 @@ otfem_status fail(otfem_solver *h, otfem_status st, const std::string &msg) {
 
 otfem_status hip_fail(otfem_solver *h, hipError_t e, const char *what) {
   return fail(h, OTFEM_HIP_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Runs body(): a failed host allocation inside it becomes
+// OTFEM_OUT_OF_MEMORY and any other C++ exception OTFEM_INVALID_ARG (with
+// the message on h when there is a handle), so no exception crosses the C
+// ABI.
+template <class F>
+otfem_status no_throw(otfem_solver *h, F &&body) {
+  auto note = [h](const char *msg) {
+    try {
+      if (h) h->err = msg;
+    } catch (...) {
+    }
+  };
+  try {
+    return body();
+  } catch (const std::bad_alloc &) {
+    note("out of host memory");
+    return OTFEM_OUT_OF_MEMORY;
+  } catch (const std::exception &e) {
+    note(e.what());
+    return OTFEM_INVALID_ARG;
+  }
 }
 
 #define HIPCHK(h, call)                                  \
@@ -606,6 +632,7 @@ const char *otfem_status_string(int status) {
     case OTFEM_HIP_ERROR: return "HIP error";
     case OTFEM_NO_DEVICE: return "no device";
     case OTFEM_STATE: return "invalid state";
+    case OTFEM_OUT_OF_MEMORY: return "out of host memory";
     default: return "unknown status";
   }
 }
@@ -650,39 +677,32 @@ otfem_status otfem_create(const otfem_mesh_desc *d, otfem_handle *out) {
   if (!d || !out || !d->coord || !d->tets || d->num_nodes <= 0 || d->num_tets < 0)
     return OTFEM_INVALID_ARG;
   *out = nullptr;
-  otfem_solver *h = new otfem_solver();
-  const int64_t N = d->num_nodes, T = d->num_tets;
-  h->x_init.assign(d->coord, d->coord + 3 * N);
-  h->x0_host.assign(d->model_coord ? d->model_coord : d->coord,
-                    (d->model_coord ? d->model_coord : d->coord) + 3 * N);
-  if (d->velocity)
-    h->v_init.assign(d->velocity, d->velocity + 3 * N);
-  else
-    h->v_init.assign(3 * N, 0.0);
-  if (d->fixed)
-    h->fixed_host.assign(d->fixed, d->fixed + N);
-  else
-    h->fixed_host.assign(N, 0);
-  for (auto &f : h->fixed_host) f = f ? 1 : 0;
-  h->tets.assign(d->tets, d->tets + 4 * T);
-  for (double c : h->x_init)
-    if (!std::isfinite(c)) {
-      delete h;
-      return OTFEM_INVALID_ARG;
-    }
-  int B = d->num_bodies > 1 ? d->num_bodies : 1;
-  if (B > 1 && (!d->body_node_offsets || !d->body_tet_offsets)) {
-    delete h;
-    return OTFEM_INVALID_ARG;
-  }
-  std::string e = build_layout(N, T, h->x0_host.data(), h->tets.data(), B, d->body_node_offsets,
-                               d->body_tet_offsets, d->target_tile_nodes, 16384, h->L);
-  if (!e.empty()) {
-    delete h;
-    return OTFEM_INVALID_ARG;
-  }
-  *out = h;
-  return OTFEM_OK;
+  return no_throw(nullptr, [&]() -> otfem_status {
+    std::unique_ptr<otfem_solver> h(new otfem_solver());
+    const int64_t N = d->num_nodes, T = d->num_tets;
+    h->x_init.assign(d->coord, d->coord + 3 * N);
+    h->x0_host.assign(d->model_coord ? d->model_coord : d->coord,
+                      (d->model_coord ? d->model_coord : d->coord) + 3 * N);
+    if (d->velocity)
+      h->v_init.assign(d->velocity, d->velocity + 3 * N);
+    else
+      h->v_init.assign(3 * N, 0.0);
+    if (d->fixed)
+      h->fixed_host.assign(d->fixed, d->fixed + N);
+    else
+      h->fixed_host.assign(N, 0);
+    for (auto &f : h->fixed_host) f = f ? 1 : 0;
+    h->tets.assign(d->tets, d->tets + 4 * T);
+    for (double c : h->x_init)
+      if (!std::isfinite(c)) return OTFEM_INVALID_ARG;
+    int B = d->num_bodies > 1 ? d->num_bodies : 1;
+    if (B > 1 && (!d->body_node_offsets || !d->body_tet_offsets)) return OTFEM_INVALID_ARG;
+    std::string e = build_layout(N, T, h->x0_host.data(), h->tets.data(), B, d->body_node_offsets,
+                                 d->body_tet_offsets, d->target_tile_nodes, 16384, h->L);
+    if (!e.empty()) return OTFEM_INVALID_ARG;
+    *out = h.release();
+    return OTFEM_OK;
+  });
 }
 
 otfem_status otfem_destroy(otfem_handle h) {
@@ -697,17 +717,28 @@ otfem_status otfem_destroy(otfem_handle h) {
 
 const char *otfem_last_error(otfem_handle h) { return h ? h->err.c_str() : "null handle"; }
 
-otfem_status otfem_init(otfem_handle h, double young, double poisson, double density,
+static otfem_status otfem_init_body(otfem_handle h, double young, double poisson, double density,
                         double c_yield, double c_creep, double c_max) {
   if (!h) return OTFEM_INVALID_ARG;
   return init_common(h, &young, &poisson, &density, true, c_yield, c_creep, c_max);
 }
 
-otfem_status otfem_init_materials(otfem_handle h, const double *young, const double *poisson,
+otfem_status otfem_init(otfem_handle h, double young, double poisson, double density,
+                        double c_yield, double c_creep, double c_max) {
+  return no_throw(h, [&] { return otfem_init_body(h, young, poisson, density, c_yield, c_creep, c_max); });
+}
+
+static otfem_status otfem_init_materials_body(otfem_handle h, const double *young, const double *poisson,
                                   const double *density, double c_yield, double c_creep,
                                   double c_max) {
   if (!h || !young || !poisson || !density) return OTFEM_INVALID_ARG;
   return init_common(h, young, poisson, density, false, c_yield, c_creep, c_max);
+}
+
+otfem_status otfem_init_materials(otfem_handle h, const double *young, const double *poisson,
+                                  const double *density, double c_yield, double c_creep,
+                                  double c_max) {
+  return no_throw(h, [&] { return otfem_init_materials_body(h, young, poisson, density, c_yield, c_creep, c_max); });
 }
 
 otfem_status otfem_set_external_forces(otfem_handle h, const double *f) {
@@ -1139,12 +1170,12 @@ static thread_local std::string g_tune_err;
 
 const char *otfem_autotune_layout_last_error(void) { return g_tune_err.c_str(); }
 
-otfem_status otfem_autotune_layout(const otfem_mesh_desc *desc, const double *young,
-                                   const double *poisson, const double *density,
-                                   const double *f_ext, double dt, int use_stiffness_warping,
-                                   const otfem_params *params, int steps,
-                                   const int32_t *candidates, int num_candidates,
-                                   int32_t *best_tile_nodes, int *best_block, double *best_ms) {
+static otfem_status autotune_layout_body(const otfem_mesh_desc *desc, const double *young,
+                                        const double *poisson, const double *density,
+                                        const double *f_ext, double dt, int use_stiffness_warping,
+                                        const otfem_params *params, int steps,
+                                        const int32_t *candidates, int num_candidates,
+                                        int32_t *best_tile_nodes, int *best_block, double *best_ms) {
   if (!desc || !young || !poisson || !density || !best_tile_nodes || num_candidates < 0 ||
       (num_candidates > 0 && !candidates) || desc->num_nodes <= 0) {
     g_tune_err = "invalid argument";
@@ -1200,6 +1231,26 @@ otfem_status otfem_autotune_layout(const otfem_mesh_desc *desc, const double *yo
   return OTFEM_OK;
 }
 
+otfem_status otfem_autotune_layout(const otfem_mesh_desc *desc, const double *young,
+                                   const double *poisson, const double *density,
+                                   const double *f_ext, double dt, int use_stiffness_warping,
+                                   const otfem_params *params, int steps,
+                                   const int32_t *candidates, int num_candidates,
+                                   int32_t *best_tile_nodes, int *best_block, double *best_ms) {
+  const otfem_status st = no_throw(nullptr, [&] {
+    return autotune_layout_body(desc, young, poisson, density, f_ext, dt, use_stiffness_warping,
+                                params, steps, candidates, num_candidates, best_tile_nodes,
+                                best_block, best_ms);
+  });
+  if (st == OTFEM_OUT_OF_MEMORY) {
+    try {
+      g_tune_err = "out of host memory";
+    } catch (...) {
+    }
+  }
+  return st;
+}
+
 otfem_status otfem_body_stats(otfem_handle h, int *iterations, double *rel_residual) {
   if (!h) return OTFEM_INVALID_ARG;
   if (h->last_body.size() != (size_t)h->L.B) return fail(h, OTFEM_STATE, "no completed step");
@@ -1228,7 +1279,7 @@ otfem_status otfem_update_original_coord(otfem_handle h) {
   return OTFEM_OK;
 }
 
-otfem_status otfem_export_K(otfem_handle h, int64_t *nnz_blocks, int64_t *row_ptr, int32_t *cols,
+static otfem_status otfem_export_K_body(otfem_handle h, int64_t *nnz_blocks, int64_t *row_ptr, int32_t *cols,
                             double *blocks) {
   if (!h || !nnz_blocks) return OTFEM_INVALID_ARG;
   if (!h->initialized) return fail(h, OTFEM_STATE, "call otfem_init first");
@@ -1306,6 +1357,11 @@ otfem_status otfem_export_K(otfem_handle h, int64_t *nnz_blocks, int64_t *row_pt
   return OTFEM_OK;
 }
 
+otfem_status otfem_export_K(otfem_handle h, int64_t *nnz_blocks, int64_t *row_ptr, int32_t *cols,
+                            double *blocks) {
+  return no_throw(h, [&] { return otfem_export_K_body(h, nnz_blocks, row_ptr, cols, blocks); });
+}
+
 otfem_status otfem_layout_info(otfem_handle h, int64_t *num_tiles, int64_t *num_slots,
                                int64_t *num_halo, int64_t *max_tile_nodes) {
   if (!h) return OTFEM_INVALID_ARG;
@@ -1359,7 +1415,7 @@ void hist_to_internal(const Layout &L, const std::vector<float> &in, std::vector
 
 }  // namespace
 
-otfem_status otfem_save_state(otfem_handle h, const char *path) {
+static otfem_status otfem_save_state_body(otfem_handle h, const char *path) {
   if (!h || !path) return OTFEM_INVALID_ARG;
   if (!h->initialized) return fail(h, OTFEM_STATE, "call otfem_init first");
   otfem_status st = ensure_device(h);
@@ -1415,7 +1471,11 @@ otfem_status otfem_save_state(otfem_handle h, const char *path) {
   return OTFEM_OK;
 }
 
-otfem_status otfem_load_state(otfem_handle h, const char *path) {
+otfem_status otfem_save_state(otfem_handle h, const char *path) {
+  return no_throw(h, [&] { return otfem_save_state_body(h, path); });
+}
+
+static otfem_status otfem_load_state_body(otfem_handle h, const char *path) {
   if (!h || !path) return OTFEM_INVALID_ARG;
   std::ifstream in(path, std::ios::binary);
   if (!in) return fail(h, OTFEM_INVALID_ARG, std::string("cannot read ") + path);
@@ -1495,9 +1555,13 @@ otfem_status otfem_load_state(otfem_handle h, const char *path) {
   return OTFEM_OK;
 }
 
+otfem_status otfem_load_state(otfem_handle h, const char *path) {
+  return no_throw(h, [&] { return otfem_load_state_body(h, path); });
+}
+
 // ---- surface coupling ----
 
-otfem_status otfem_surface_bind(otfem_handle h, int64_t num_vertices, const double *vertices,
+static otfem_status otfem_surface_bind_body(otfem_handle h, int64_t num_vertices, const double *vertices,
                                 double *barycentric, int32_t *tet_index) {
   if (!h) return OTFEM_INVALID_ARG;
   if (num_vertices < 0 || (num_vertices > 0 && (!vertices || !barycentric || !tet_index)))
@@ -1513,6 +1577,11 @@ otfem_status otfem_surface_bind(otfem_handle h, int64_t num_vertices, const doub
   coupling::bind_points(h->L.N, x.data(), h->L.T, h->tets.data(), num_vertices, vertices,
                         barycentric, tet_index);
   return OTFEM_OK;
+}
+
+otfem_status otfem_surface_bind(otfem_handle h, int64_t num_vertices, const double *vertices,
+                                double *barycentric, int32_t *tet_index) {
+  return no_throw(h, [&] { return otfem_surface_bind_body(h, num_vertices, vertices, barycentric, tet_index); });
 }
 
 struct otfem_surface {
@@ -1548,7 +1617,7 @@ void surface_free(otfem_surface *s) {
       return surface_fail((s), OTFEM_HIP_ERROR, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-otfem_status otfem_surface_create(otfem_handle h, int64_t num_vertices, const double *vertices,
+static otfem_status otfem_surface_create_body(otfem_handle h, int64_t num_vertices, const double *vertices,
                                   const double *barycentric, const int32_t *tet_index,
                                   otfem_surface_handle *out) {
   if (!h || !out) return OTFEM_INVALID_ARG;
@@ -1608,6 +1677,12 @@ otfem_status otfem_surface_create(otfem_handle h, int64_t num_vertices, const do
   if (e != OTFEM_OK) return bail(e);
   *out = s;
   return OTFEM_OK;
+}
+
+otfem_status otfem_surface_create(otfem_handle h, int64_t num_vertices, const double *vertices,
+                                  const double *barycentric, const int32_t *tet_index,
+                                  otfem_surface_handle *out) {
+  return no_throw(h, [&] { return otfem_surface_create_body(h, num_vertices, vertices, barycentric, tet_index, out); });
 }
 
 otfem_status otfem_surface_update(otfem_surface_handle s, double *vertices_out, double *kernel_ms) {
@@ -1757,7 +1832,7 @@ static otfem_status residual_operator(otfem_solver *h, double dt, const otfem_pa
   return OTFEM_OK;
 }
 
-otfem_status otfem_model_train_residual(otfem_model_handle m, otfem_handle h, double dt,
+static otfem_status otfem_model_train_residual_body(otfem_model_handle m, otfem_handle h, double dt,
                                         const otfem_params *params, int steps,
                                         int steps_per_batch, double lr, uint64_t seed,
                                         float *loss) {
@@ -1771,7 +1846,14 @@ otfem_status otfem_model_train_residual(otfem_model_handle m, otfem_handle h, do
   return st;
 }
 
-otfem_status otfem_model_residual_gradient(otfem_model_handle m, otfem_handle h, double dt,
+otfem_status otfem_model_train_residual(otfem_model_handle m, otfem_handle h, double dt,
+                                        const otfem_params *params, int steps,
+                                        int steps_per_batch, double lr, uint64_t seed,
+                                        float *loss) {
+  return no_throw(h, [&] { return otfem_model_train_residual_body(m, h, dt, params, steps, steps_per_batch, lr, seed, loss); });
+}
+
+static otfem_status otfem_model_residual_gradient_body(otfem_model_handle m, otfem_handle h, double dt,
                                            const otfem_params *params, const int32_t *steps,
                                            int num_steps, double *loss, float *grad) {
   if (!m || !h || !steps || num_steps <= 0) return OTFEM_INVALID_ARG;
@@ -1782,6 +1864,12 @@ otfem_status otfem_model_residual_gradient(otfem_model_handle m, otfem_handle h,
   st = (otfem_status)model_residual(m, op, 1, num_steps, 1.0, 0, nullptr, steps, grad, loss, &err);
   if (st) h->err = err;
   return st;
+}
+
+otfem_status otfem_model_residual_gradient(otfem_model_handle m, otfem_handle h, double dt,
+                                           const otfem_params *params, const int32_t *steps,
+                                           int num_steps, double *loss, float *grad) {
+  return no_throw(h, [&] { return otfem_model_residual_gradient_body(m, h, dt, params, steps, num_steps, loss, grad); });
 }
 
 }  // extern "C"

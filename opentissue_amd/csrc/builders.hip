@@ -28,6 +28,8 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -322,9 +324,9 @@ otfem_status otfem_blocks_count(const otfem_blocks_desc *d, int64_t *num_nodes, 
   return OTFEM_OK;
 }
 
-otfem_status otfem_blocks_build(const otfem_blocks_desc *d, double *coords, int32_t *tets,
-                                uint8_t *fixed, double *f_ext, int64_t *body_node_offsets,
-                                int64_t *body_tet_offsets, int32_t *rounds) {
+static otfem_status blocks_build(const otfem_blocks_desc *d, double *coords, int32_t *tets,
+                                 uint8_t *fixed, double *f_ext, int64_t *body_node_offsets,
+                                 int64_t *body_tet_offsets, int32_t *rounds) {
   int64_t N = 0, C = 0;
   otfem_status st = check_desc(d, &N, &C);
   if (st != OTFEM_OK) return st;
@@ -468,6 +470,27 @@ otfem_status otfem_blocks_build(const otfem_blocks_desc *d, double *coords, int3
       !hip(hipStreamSynchronize(g.s), "sync"))
     return OTFEM_HIP_ERROR;
   return OTFEM_OK;
+}
+
+otfem_status otfem_blocks_build(const otfem_blocks_desc *d, double *coords, int32_t *tets,
+                                uint8_t *fixed, double *f_ext, int64_t *body_node_offsets,
+                                int64_t *body_tet_offsets, int32_t *rounds) {
+  // no C++ exception crosses the C ABI
+  try {
+    return blocks_build(d, coords, tets, fixed, f_ext, body_node_offsets, body_tet_offsets, rounds);
+  } catch (const std::bad_alloc &) {
+    try {
+      g_err = "out of host memory";
+    } catch (...) {
+    }
+    return OTFEM_OUT_OF_MEMORY;
+  } catch (const std::exception &e) {
+    try {
+      g_err = e.what();
+    } catch (...) {
+    }
+    return OTFEM_INVALID_ARG;
+  }
 }
 
 }  // extern "C"

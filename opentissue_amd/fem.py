@@ -27,7 +27,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libotfem.so")
 _lib = None
 
-OK, INVALID_ARG, INVERTED_TET, NONFINITE, NOT_CONVERGED, HIP_ERROR, NO_DEVICE, STATE = range(8)
+OK, INVALID_ARG, INVERTED_TET, NONFINITE, NOT_CONVERGED, HIP_ERROR, NO_DEVICE, STATE, OUT_OF_MEMORY = range(9)
 
 
 class OtfemError(RuntimeError):
@@ -300,7 +300,8 @@ class Solver:
         st = self.lib.otfem_create(C.byref(d), C.byref(h))  # copies what d points to
         del keep
         if st != OK:
-            raise OtfemError(st, "otfem_create failed (invalid mesh description)")
+            raise OtfemError(st, "otfem_create failed: " + ("out of host memory" if st == OUT_OF_MEMORY
+                                                           else "invalid mesh description"))
         self.h = h
 
     def close(self):

@@ -155,3 +155,12 @@ def test_blocks_counts_and_argument_checks(fem):
         with pytest.raises(fem.OtfemError) as e:
             fem.build_blocks([(2, 2, 2)], 0.01)
         assert e.value.status == fem.NO_DEVICE
+
+
+def test_status_strings(fem):
+    lib = fem.load_library()
+    names = {fem.OK: b"ok", fem.INVALID_ARG: b"invalid argument", fem.STATE: b"invalid state",
+             fem.OUT_OF_MEMORY: b"out of host memory"}
+    for code, text in names.items():
+        assert lib.otfem_status_string(code) == text
+    assert lib.otfem_status_string(99) == b"unknown status"
