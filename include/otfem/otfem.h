@@ -337,6 +337,15 @@ otfem_status otfem_surface_create(otfem_handle h, int64_t num_vertices, const do
  * the solver's queued steps, into vertices_out[3n] (caller frame).
  * kernel_ms (may be NULL) receives the update kernel's time. */
 otfem_status otfem_surface_update(otfem_surface_handle s, double *vertices_out, double *kernel_ms);
+/* Vertex normals carried by the element rotations, n = Re n0, as the coupling
+ * notes of t4mesh_mesh_coupling.h:171-176 describe (the reference leaves it to
+ * the caller): set the rest normals n0 (normals[3n], caller frame) once, then
+ * otfem_surface_update_normals writes Re n0 for every bound vertex, Re being
+ * its tetrahedron's rotation from the last warped step taken with
+ * store_rotations = 1 (Re = I before one; unbound vertices keep n0), into
+ * normals_out[3n].  No reference counterpart as a function. */
+otfem_status otfem_surface_set_normals(otfem_surface_handle s, const double *normals);
+otfem_status otfem_surface_update_normals(otfem_surface_handle s, double *normals_out);
 const char *otfem_surface_last_error(otfem_surface_handle s);
 void otfem_surface_destroy(otfem_surface_handle s);
 

@@ -1591,6 +1591,10 @@ struct otfem_surface {
   double4 *w = nullptr;     // [n] w0..w3
   double *rest = nullptr;   // [3n] positions kept for unbound vertices
   double *out = nullptr;    // [3n]
+  int32_t *tet = nullptr;   // [n] caller tetrahedron index, -1: unbound
+  double *n0 = nullptr;     // [3n] rest normals (otfem_surface_set_normals)
+  double *nout = nullptr;   // [3n]
+  bool has_normals = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
 };
@@ -1605,6 +1609,9 @@ void surface_free(otfem_surface *s) {
   if (s->w) (void)hipFree(s->w);
   if (s->rest) (void)hipFree(s->rest);
   if (s->out) (void)hipFree(s->out);
+  if (s->tet) (void)hipFree(s->tet);
+  if (s->n0) (void)hipFree(s->n0);
+  if (s->nout) (void)hipFree(s->nout);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
 }
@@ -1666,7 +1673,9 @@ static otfem_status otfem_surface_create_body(otfem_handle h, int64_t num_vertic
     step(hipMalloc(&s->w, n * sizeof(double4)), "hipMalloc");
     step(hipMalloc(&s->rest, 3 * n * sizeof(double)), "hipMalloc");
     step(hipMalloc(&s->out, 3 * n * sizeof(double)), "hipMalloc");
+    step(hipMalloc(&s->tet, n * sizeof(int32_t)), "hipMalloc");
     if (e == OTFEM_OK) {
+      step(hipMemcpy(s->tet, tet_index, n * sizeof(int32_t), hipMemcpyHostToDevice), "hipMemcpy");
       step(hipMemcpy(s->ids, ids.data(), n * sizeof(int4), hipMemcpyHostToDevice), "hipMemcpy");
       step(hipMemcpy(s->w, w.data(), n * sizeof(double4), hipMemcpyHostToDevice), "hipMemcpy");
       step(hipMemcpy(s->rest, vertices, 3 * n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
@@ -1702,6 +1711,35 @@ otfem_status otfem_surface_update(otfem_surface_handle s, double *vertices_out, 
     SCHK(s, hipEventElapsedTime(&ms, s->ev0, s->ev1));
     *kernel_ms = ms;
   }
+  return OTFEM_OK;
+}
+
+otfem_status otfem_surface_set_normals(otfem_surface_handle s, const double *normals) {
+  if (!s) return OTFEM_INVALID_ARG;
+  if (s->n > 0 && !normals) return surface_fail(s, OTFEM_INVALID_ARG, "normals is NULL");
+  if (s->n > 0) {
+    if (!s->n0) {
+      SCHK(s, hipMalloc(&s->n0, 3 * s->n * sizeof(double)));
+      SCHK(s, hipMalloc(&s->nout, 3 * s->n * sizeof(double)));
+    }
+    SCHK(s, hipMemcpyAsync(s->n0, normals, 3 * s->n * sizeof(double), hipMemcpyHostToDevice,
+                           s->solver->stream));
+    SCHK(s, hipStreamSynchronize(s->solver->stream));
+  }
+  s->has_normals = true;
+  return OTFEM_OK;
+}
+
+otfem_status otfem_surface_update_normals(otfem_surface_handle s, double *normals_out) {
+  if (!s) return OTFEM_INVALID_ARG;
+  if (!s->has_normals) return surface_fail(s, OTFEM_STATE, "call otfem_surface_set_normals first");
+  if (s->n > 0 && !normals_out) return surface_fail(s, OTFEM_INVALID_ARG, "normals_out is NULL");
+  otfem_solver *h = s->solver;
+  hipStream_t st = h->stream;
+  SCHK(s, enqueue_surface_normals(s->n, s->tet, h->Re, h->rotations_valid ? 1 : 0, s->n0, s->nout, st));
+  if (s->n > 0)
+    SCHK(s, hipMemcpyAsync(normals_out, s->nout, 3 * s->n * sizeof(double), hipMemcpyDeviceToHost, st));
+  SCHK(s, hipStreamSynchronize(st));
   return OTFEM_OK;
 }
 

@@ -105,6 +105,10 @@ hipError_t enqueue_finish(const StepArgs &a, const LaunchConfig &lc, hipStream_t
 // vertices (ids[i].x >= 0, internal node ids), rest[3i..] for the others.
 hipError_t enqueue_surface_update(int64_t n, const int4 *ids, const double4 *w, const double *rest,
                                   const double *x, double *out, hipStream_t s);
+// Surface normals: out[3i..] = Re[tet[i]] n0[3i..] (Re row-major [9T]) for
+// bound vertices when use_re, n0 otherwise.
+hipError_t enqueue_surface_normals(int64_t n, const int32_t *tet, const double *Re, int use_re,
+                                   const double *n0, double *out, hipStream_t s);
 size_t element_lds_bytes(int max_nodes, int max_owned, int plastic_on);
 size_t cg_lds_bytes(int max_nodes, int max_owned);
 

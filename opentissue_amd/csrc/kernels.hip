@@ -1132,6 +1132,42 @@ __global__ __launch_bounds__(kSurfaceBlock) void k_surface_update(int64_t n, con
 
 }  // namespace
 
+namespace {
+
+// Vertex normals carried by the element rotations, as the coupling notes of
+// t4mesh_mesh_coupling.h:171-176 describe (n = Re n0): Re of the vertex's
+// tetrahedron (row-major, caller tetrahedron order) times the rest normal;
+// unbound vertices (tet < 0), or all of them when no rotations are stored
+// (Re = I), keep n0.
+__global__ __launch_bounds__(kSurfaceBlock) void k_surface_normals(int64_t n, const int32_t *tet,
+                                                                   const double *Re, int use_re,
+                                                                   const double *n0, double *out) {
+  const int64_t i = (int64_t)blockIdx.x * kSurfaceBlock + threadIdx.x;
+  if (i >= n) return;
+  const double a = n0[3 * i], b = n0[3 * i + 1], c = n0[3 * i + 2];
+  const int32_t t = tet[i];
+  if (t < 0 || !use_re) {
+    out[3 * i] = a;
+    out[3 * i + 1] = b;
+    out[3 * i + 2] = c;
+    return;
+  }
+  const double *R = Re + 9 * (int64_t)t;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) out[3 * i + r] = (R[3 * r] * a + R[3 * r + 1] * b) + R[3 * r + 2] * c;
+}
+
+}  // namespace
+
+hipError_t enqueue_surface_normals(int64_t n, const int32_t *tet, const double *Re, int use_re,
+                                   const double *n0, double *out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int64_t blocks = (n + kSurfaceBlock - 1) / kSurfaceBlock;
+  hipLaunchKernelGGL(k_surface_normals, dim3((unsigned)blocks), dim3(kSurfaceBlock), 0, s, n, tet, Re,
+                     use_re, n0, out);
+  return hipGetLastError();
+}
+
 hipError_t enqueue_surface_update(int64_t n, const int4 *ids, const double4 *w, const double *rest,
                                   const double *x, double *out, hipStream_t s) {
   if (n <= 0) return hipSuccess;

@@ -176,6 +176,8 @@ SIGNATURES = {
     "otfem_surface_create": (C.c_int, [_H, C.c_int64, _dp, _dp, C.POINTER(C.c_int32),
                                        C.POINTER(C.c_void_p)]),
     "otfem_surface_update": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "otfem_surface_set_normals": (C.c_int, [C.c_void_p, _dp]),
+    "otfem_surface_update_normals": (C.c_int, [C.c_void_p, _dp]),
     "otfem_surface_last_error": (C.c_char_p, [C.c_void_p]),
     "otfem_surface_destroy": (None, [C.c_void_p]),
     "otfem_blocks_count": (C.c_int, [C.POINTER(BlocksDesc), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
@@ -524,6 +526,23 @@ class Surface:
             msg = self.lib.otfem_surface_last_error(self.h)
             raise OtfemError(st, msg.decode() if msg else "")
         self.kernel_ms = ms.value
+        return out
+
+    def _surface_check(self, st):
+        if st != OK:
+            msg = self.lib.otfem_surface_last_error(self.h)
+            raise OtfemError(st, msg.decode() if msg else "")
+
+    def set_normals(self, normals):
+        """Rest normals n0 [n, 3] of the vertices."""
+        n0 = _f64(normals, (self.n, 3))
+        self._surface_check(self.lib.otfem_surface_set_normals(self.h, _dptr(n0)))
+
+    def update_normals(self):
+        """n = Re n0 per bound vertex (t4mesh_mesh_coupling.h:171-176), Re of
+        its tetrahedron from the last warped step with store_rotations."""
+        out = np.zeros((self.n, 3))
+        self._surface_check(self.lib.otfem_surface_update_normals(self.h, _dptr(out)))
         return out
 
     def close(self):

@@ -10,12 +10,16 @@ Follows core/containers/t4mesh/util/t4mesh_mesh_coupling.h:
             tetrahedra in index order, so the first hit is the lowest index
     update  :202-238  (w0 = 1 - w1 - w2 - w3; p0 w0 + p1 w1 + p2 w2 + p3 w3)
 and core/geometry/geometry_barycentric.h:339-372 for the coordinates
-(w = P^-1 (p - x4), P = [x1 - x4, x2 - x4, x3 - x4]).
+(w = P^-1 (p - x4), P = [x1 - x4, x2 - x4, x3 - x4]), and the normal update
+the coupling notes describe (:171-176): n = Re n0 with Re the bound
+tetrahedron's rotation.
 
 Pins (tests/test_coupling.py): corners and centroids of tetrahedra have the
 closed-form coordinates, a bound point is reproduced by the update at the
 binding positions, an affine motion of the nodes moves every bound point by
-the same affine map, and points outside the mesh stay unbound.
+the same affine map, and points outside the mesh stay unbound; for normals,
+identity rotations keep n0 and a rotation Q gives Q n0 (a closed form the
+GPU test also checks on a rigidly rotated mesh).
 """
 from __future__ import annotations
 
@@ -69,4 +73,15 @@ def update(nodes, tets, w123, idx, points):
         w0 = 1.0 - w1 - w2 - w3
         c = np.asarray(tets)[t]
         out[i] = ((nodes[c[0]] * w0 + nodes[c[1]] * w1) + nodes[c[2]] * w2) + nodes[c[3]] * w3
+    return out
+
+
+def normals(Re, idx, n0):
+    """n = Re[idx] n0 per bound vertex (Re [T, 3, 3] row-major), n0 for
+    unbound ones (idx -1)."""
+    Re = np.asarray(Re, dtype=np.float64).reshape(-1, 3, 3)
+    n0 = np.asarray(n0, dtype=np.float64).reshape(-1, 3)
+    out = n0.copy()
+    b = np.asarray(idx) >= 0
+    out[b] = np.einsum("nij,nj->ni", Re[np.asarray(idx)[b]], n0[b])
     return out

@@ -73,3 +73,16 @@ def test_cpp_header_matches_oracle(tmp_path):
     bound = idx >= 0
     assert np.abs(out[bound, 4:] - (pts[bound] @ A.T + SHIFT)).max() < 1e-12
     assert np.array_equal(out[~bound, 4:], pts[~bound])
+
+
+def test_oracle_normals_closed_forms():
+    rng = np.random.default_rng(4)
+    n0 = rng.normal(size=(50, 3))
+    idx = rng.integers(-1, 6, 50)
+    eye = np.broadcast_to(np.eye(3), (6, 3, 3))
+    assert np.array_equal(OC.normals(eye, idx, n0), n0)
+    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    out = OC.normals(np.broadcast_to(Q, (6, 3, 3)), idx, n0)
+    b = idx >= 0
+    assert np.allclose(out[b], n0[b] @ Q.T, atol=1e-14)
+    assert np.array_equal(out[~b], n0[~b])
