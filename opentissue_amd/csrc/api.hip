@@ -718,19 +718,21 @@ otfem_status otfem_destroy(otfem_handle h) {
 const char *otfem_last_error(otfem_handle h) { return h ? h->err.c_str() : "null handle"; }
 
 static otfem_status otfem_init_body(otfem_handle h, double young, double poisson, double density,
-                        double c_yield, double c_creep, double c_max) {
+                                    double c_yield, double c_creep, double c_max) {
   if (!h) return OTFEM_INVALID_ARG;
   return init_common(h, &young, &poisson, &density, true, c_yield, c_creep, c_max);
 }
 
 otfem_status otfem_init(otfem_handle h, double young, double poisson, double density,
                         double c_yield, double c_creep, double c_max) {
-  return no_throw(h, [&] { return otfem_init_body(h, young, poisson, density, c_yield, c_creep, c_max); });
+  return no_throw(h, [&] {
+    return otfem_init_body(h, young, poisson, density, c_yield, c_creep, c_max);
+  });
 }
 
 static otfem_status otfem_init_materials_body(otfem_handle h, const double *young, const double *poisson,
-                                  const double *density, double c_yield, double c_creep,
-                                  double c_max) {
+                                              const double *density, double c_yield, double c_creep,
+                                              double c_max) {
   if (!h || !young || !poisson || !density) return OTFEM_INVALID_ARG;
   return init_common(h, young, poisson, density, false, c_yield, c_creep, c_max);
 }
@@ -738,7 +740,9 @@ static otfem_status otfem_init_materials_body(otfem_handle h, const double *youn
 otfem_status otfem_init_materials(otfem_handle h, const double *young, const double *poisson,
                                   const double *density, double c_yield, double c_creep,
                                   double c_max) {
-  return no_throw(h, [&] { return otfem_init_materials_body(h, young, poisson, density, c_yield, c_creep, c_max); });
+  return no_throw(h, [&] {
+    return otfem_init_materials_body(h, young, poisson, density, c_yield, c_creep, c_max);
+  });
 }
 
 otfem_status otfem_set_external_forces(otfem_handle h, const double *f) {
@@ -1104,18 +1108,24 @@ static otfem_status tune_blocks(otfem_handle h, double dt, int use_stiffness_war
   const int64_t N = h->L.N, T = h->L.T;
   // the trial steps must leave the caller's state untouched
   double *sx = nullptr, *sv = nullptr, *sf = nullptr, *spl = nullptr;
-  HIPCHK(h, dalloc(&sx, 3 * N));
-  HIPCHK(h, dalloc(&sv, 3 * N));
-  HIPCHK(h, dalloc(&sf, 3 * N));
-  HIPCHK(h, dalloc(&spl, 6 * T));
+  hipError_t ae = dalloc(&sx, 3 * N);
+  if (ae == hipSuccess) ae = dalloc(&sv, 3 * N);
+  if (ae == hipSuccess) ae = dalloc(&sf, 3 * N);
+  if (ae == hipSuccess) ae = dalloc(&spl, 6 * T);
+  if (ae != hipSuccess) {
+    dfree(sx); dfree(sv); dfree(sf); dfree(spl);
+    return hip_fail(h, ae, "autotune: state copies");
+  }
   hipStream_t s = h->stream;
   auto save = [&](bool restore) -> hipError_t {
-    hipMemcpyKind k = hipMemcpyDeviceToDevice;
+    auto copy = [&](double *live, double *kept, int64_t count) {
+      return hipMemcpyAsync(restore ? live : kept, restore ? kept : live, count * sizeof(double),
+                            hipMemcpyDeviceToDevice, s);
+    };
     hipError_t e;
-    if ((e = hipMemcpyAsync(restore ? h->x : sx, restore ? sx : h->x, 3 * N * sizeof(double), k, s))) return e;
-    if ((e = hipMemcpyAsync(restore ? h->v : sv, restore ? sv : h->v, 3 * N * sizeof(double), k, s))) return e;
-    if ((e = hipMemcpyAsync(restore ? h->fext : sf, restore ? sf : h->fext, 3 * N * sizeof(double), k, s))) return e;
-    if ((e = hipMemcpyAsync(restore ? h->plastic : spl, restore ? spl : h->plastic, 6 * T * sizeof(double), k, s))) return e;
+    if ((e = copy(h->x, sx, 3 * N)) || (e = copy(h->v, sv, 3 * N)) || (e = copy(h->fext, sf, 3 * N)) ||
+        (e = copy(h->plastic, spl, 6 * T)))
+      return e;
     return hipStreamSynchronize(s);
   };
   // trials use the plain CG start and leave the learned-start history alone
@@ -1280,7 +1290,7 @@ otfem_status otfem_update_original_coord(otfem_handle h) {
 }
 
 static otfem_status otfem_export_K_body(otfem_handle h, int64_t *nnz_blocks, int64_t *row_ptr, int32_t *cols,
-                            double *blocks) {
+                                        double *blocks) {
   if (!h || !nnz_blocks) return OTFEM_INVALID_ARG;
   if (!h->initialized) return fail(h, OTFEM_STATE, "call otfem_init first");
   const int64_t N = h->L.N, T = h->L.T;
@@ -1562,7 +1572,7 @@ otfem_status otfem_load_state(otfem_handle h, const char *path) {
 // ---- surface coupling ----
 
 static otfem_status otfem_surface_bind_body(otfem_handle h, int64_t num_vertices, const double *vertices,
-                                double *barycentric, int32_t *tet_index) {
+                                            double *barycentric, int32_t *tet_index) {
   if (!h) return OTFEM_INVALID_ARG;
   if (num_vertices < 0 || (num_vertices > 0 && (!vertices || !barycentric || !tet_index)))
     return fail(h, OTFEM_INVALID_ARG, "surface arrays missing or negative vertex count");
@@ -1581,7 +1591,9 @@ static otfem_status otfem_surface_bind_body(otfem_handle h, int64_t num_vertices
 
 otfem_status otfem_surface_bind(otfem_handle h, int64_t num_vertices, const double *vertices,
                                 double *barycentric, int32_t *tet_index) {
-  return no_throw(h, [&] { return otfem_surface_bind_body(h, num_vertices, vertices, barycentric, tet_index); });
+  return no_throw(h, [&] {
+    return otfem_surface_bind_body(h, num_vertices, vertices, barycentric, tet_index);
+  });
 }
 
 struct otfem_surface {
@@ -1625,8 +1637,8 @@ void surface_free(otfem_surface *s) {
   } while (0)
 
 static otfem_status otfem_surface_create_body(otfem_handle h, int64_t num_vertices, const double *vertices,
-                                  const double *barycentric, const int32_t *tet_index,
-                                  otfem_surface_handle *out) {
+                                              const double *barycentric, const int32_t *tet_index,
+                                              otfem_surface_handle *out) {
   if (!h || !out) return OTFEM_INVALID_ARG;
   *out = nullptr;
   if (num_vertices < 0 || (num_vertices > 0 && (!vertices || !barycentric || !tet_index)))
@@ -1691,7 +1703,9 @@ static otfem_status otfem_surface_create_body(otfem_handle h, int64_t num_vertic
 otfem_status otfem_surface_create(otfem_handle h, int64_t num_vertices, const double *vertices,
                                   const double *barycentric, const int32_t *tet_index,
                                   otfem_surface_handle *out) {
-  return no_throw(h, [&] { return otfem_surface_create_body(h, num_vertices, vertices, barycentric, tet_index, out); });
+  return no_throw(h, [&] {
+    return otfem_surface_create_body(h, num_vertices, vertices, barycentric, tet_index, out);
+  });
 }
 
 otfem_status otfem_surface_update(otfem_surface_handle s, double *vertices_out, double *kernel_ms) {
@@ -1871,9 +1885,9 @@ static otfem_status residual_operator(otfem_solver *h, double dt, const otfem_pa
 }
 
 static otfem_status otfem_model_train_residual_body(otfem_model_handle m, otfem_handle h, double dt,
-                                        const otfem_params *params, int steps,
-                                        int steps_per_batch, double lr, uint64_t seed,
-                                        float *loss) {
+                                                    const otfem_params *params, int steps,
+                                                    int steps_per_batch, double lr, uint64_t seed,
+                                                    float *loss) {
   if (!m || !h) return OTFEM_INVALID_ARG;
   ResidualOp op;
   otfem_status st = residual_operator(h, dt, params, op);
@@ -1888,12 +1902,14 @@ otfem_status otfem_model_train_residual(otfem_model_handle m, otfem_handle h, do
                                         const otfem_params *params, int steps,
                                         int steps_per_batch, double lr, uint64_t seed,
                                         float *loss) {
-  return no_throw(h, [&] { return otfem_model_train_residual_body(m, h, dt, params, steps, steps_per_batch, lr, seed, loss); });
+  return no_throw(h, [&] {
+    return otfem_model_train_residual_body(m, h, dt, params, steps, steps_per_batch, lr, seed, loss);
+  });
 }
 
 static otfem_status otfem_model_residual_gradient_body(otfem_model_handle m, otfem_handle h, double dt,
-                                           const otfem_params *params, const int32_t *steps,
-                                           int num_steps, double *loss, float *grad) {
+                                                       const otfem_params *params, const int32_t *steps,
+                                                       int num_steps, double *loss, float *grad) {
   if (!m || !h || !steps || num_steps <= 0) return OTFEM_INVALID_ARG;
   ResidualOp op;
   otfem_status st = residual_operator(h, dt, params, op);
@@ -1907,7 +1923,9 @@ static otfem_status otfem_model_residual_gradient_body(otfem_model_handle m, otf
 otfem_status otfem_model_residual_gradient(otfem_model_handle m, otfem_handle h, double dt,
                                            const otfem_params *params, const int32_t *steps,
                                            int num_steps, double *loss, float *grad) {
-  return no_throw(h, [&] { return otfem_model_residual_gradient_body(m, h, dt, params, steps, num_steps, loss, grad); });
+  return no_throw(h, [&] {
+    return otfem_model_residual_gradient_body(m, h, dt, params, steps, num_steps, loss, grad);
+  });
 }
 
 }  // extern "C"

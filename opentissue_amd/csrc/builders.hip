@@ -291,6 +291,10 @@ struct Build {
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
+otfem_status alloc_failed(const Build &g) {
+  return fail(OTFEM_HIP_ERROR, std::string("hipMalloc: ") + hipGetErrorString(g.err));
+}
+
 otfem_status check_desc(const otfem_blocks_desc *d, int64_t *N, int64_t *C) {
   if (!d) return fail(OTFEM_INVALID_ARG, "null description");
   if (d->num_bodies < 1 || !d->dims) return fail(OTFEM_INVALID_ARG, "need num_bodies >= 1 and dims");
@@ -302,7 +306,8 @@ otfem_status check_desc(const otfem_blocks_desc *d, int64_t *N, int64_t *C) {
     if (I < 1 || J < 1 || K < 1) return fail(OTFEM_INVALID_ARG, "every body needs I, J, K >= 1");
     n += (I + 1) * (J + 1) * (K + 1);
     c += I * J * K;
-    if (n >= INT32_MAX || 5 * c >= INT32_MAX) return fail(OTFEM_INVALID_ARG, "too many nodes or tetrahedra for 32-bit ids");
+    if (n >= INT32_MAX || 5 * c >= INT32_MAX)
+      return fail(OTFEM_INVALID_ARG, "too many nodes or tetrahedra for 32-bit ids");
   }
   *N = n;
   *C = c;
@@ -370,7 +375,7 @@ static otfem_status blocks_build(const otfem_blocks_desc *d, double *coords, int
   double *x0 = g.alloc<double>(3 * N), *x = g.alloc<double>(3 * N);
   uint8_t *interior = g.alloc<uint8_t>(N), *fx = g.alloc<uint8_t>(N), *mark = g.alloc<uint8_t>(N);
   int32_t *tt = g.alloc<int32_t>(4 * T);
-  if (g.err != hipSuccess) return fail(OTFEM_HIP_ERROR, std::string("hipMalloc: ") + hipGetErrorString(g.err));
+  if (g.err != hipSuccess) return alloc_failed(g);
   auto hip = [&](hipError_t e, const char *what) {
     if (e != hipSuccess && g.err == hipSuccess) g.err = e;
     if (e != hipSuccess) g_err = std::string(what) + ": " + hipGetErrorString(e);
@@ -399,7 +404,7 @@ static otfem_status blocks_build(const otfem_blocks_desc *d, double *coords, int
              "select size"))
       return OTFEM_HIP_ERROR;
     void *tmp = g.alloc<char>(tmp_bytes);
-    if (g.err != hipSuccess) return fail(OTFEM_HIP_ERROR, std::string("hipMalloc: ") + hipGetErrorString(g.err));
+    if (g.err != hipSuccess) return alloc_failed(g);
     std::vector<double> hdisp;
     const double amp[3] = {d->jitter * d->width, d->jitter * d->height, d->jitter * d->depth};
     // Round 0 moves every interior node; round r >= 1 tests every
@@ -423,7 +428,8 @@ static otfem_status blocks_build(const otfem_blocks_desc *d, double *coords, int
       int32_t n = 0;
       unsigned long long bad = 0;
       if (!hip(hipMemcpyAsync(&n, n_sel, sizeof(n), hipMemcpyDeviceToHost, g.s), "copy count") ||
-          (round > 0 && !hip(hipMemcpyAsync(&bad, n_bad, sizeof(bad), hipMemcpyDeviceToHost, g.s), "copy count")) ||
+          (round > 0 &&
+           !hip(hipMemcpyAsync(&bad, n_bad, sizeof(bad), hipMemcpyDeviceToHost, g.s), "copy count")) ||
           !hip(hipStreamSynchronize(g.s), "sync"))
         return OTFEM_HIP_ERROR;
       if (round > 0) {
@@ -432,18 +438,22 @@ static otfem_status blocks_build(const otfem_blocks_desc *d, double *coords, int
           break;
         }
         if (n == 0)
-          return fail(OTFEM_INVALID_ARG, "jitter: a tetrahedron with no interior node is below the volume floor");
+          return fail(OTFEM_INVALID_ARG,
+                      "jitter: a tetrahedron with no interior node is below the volume floor");
         if (round == d->max_rounds) break;
       }
       if (n == 0) continue;  // round 0 without interior nodes
       if (d->draw) {
         hdisp.resize(3 * (size_t)n);
-        if (d->draw(d->draw_user, n, hdisp.data()) != 0) return fail(OTFEM_INVALID_ARG, "jitter: the random source failed");
-        if (!hip(hipMemcpyAsync(disp, hdisp.data(), 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, g.s),
+        if (d->draw(d->draw_user, n, hdisp.data()) != 0)
+          return fail(OTFEM_INVALID_ARG, "jitter: the random source failed");
+        if (!hip(hipMemcpyAsync(disp, hdisp.data(), 3 * (size_t)n * sizeof(double),
+                                hipMemcpyHostToDevice, g.s),
                  "copy displacements"))
           return OTFEM_HIP_ERROR;
       } else {
-        k_builtin_draw<<<grid_for(n, blk), blk, 0, g.s>>>(list, n, d->seed, round, amp[0], amp[1], amp[2], disp);
+        k_builtin_draw<<<grid_for(n, blk), blk, 0, g.s>>>(list, n, d->seed, round, amp[0], amp[1],
+                                                          amp[2], disp);
       }
       k_move<<<grid_for(n, blk), blk, 0, g.s>>>(list, n, x0, disp, x);
       if (!hip(hipGetLastError(), "move")) return OTFEM_HIP_ERROR;
@@ -458,7 +468,7 @@ static otfem_status blocks_build(const otfem_blocks_desc *d, double *coords, int
   double *f = nullptr;
   if (f_ext) {
     f = g.alloc<double>(3 * N);
-    if (g.err != hipSuccess) return fail(OTFEM_HIP_ERROR, std::string("hipMalloc: ") + hipGetErrorString(g.err));
+    if (g.err != hipSuccess) return alloc_failed(g);
     k_face_load<<<grid_for(N, blk), blk, 0, g.s>>>(bd, (int)B, N, x, has_load ? d->load_axis : -1,
                                                   d->load_side, fx, f);
     if (!hip(hipGetLastError(), "face load")) return OTFEM_HIP_ERROR;
@@ -466,7 +476,8 @@ static otfem_status blocks_build(const otfem_blocks_desc *d, double *coords, int
   if (!hip(hipMemcpyAsync(coords, x, 3 * N * sizeof(double), hipMemcpyDeviceToHost, g.s), "copy coords") ||
       !hip(hipMemcpyAsync(tets, tt, 4 * T * sizeof(int32_t), hipMemcpyDeviceToHost, g.s), "copy tets") ||
       (fixed && !hip(hipMemcpyAsync(fixed, fx, N, hipMemcpyDeviceToHost, g.s), "copy fixed")) ||
-      (f_ext && !hip(hipMemcpyAsync(f_ext, f, 3 * N * sizeof(double), hipMemcpyDeviceToHost, g.s), "copy forces")) ||
+      (f_ext &&
+       !hip(hipMemcpyAsync(f_ext, f, 3 * N * sizeof(double), hipMemcpyDeviceToHost, g.s), "copy forces")) ||
       !hip(hipStreamSynchronize(g.s), "sync"))
     return OTFEM_HIP_ERROR;
   return OTFEM_OK;
