@@ -277,11 +277,26 @@ def main():
     log(f"timed {args.steps} steps in {wall:.3f} s; element {elem_ms:.2f} ms, CG {cg_ms:.2f} ms; "
         f"CG iterations {total_iters} ({total_iters / max(1, args.steps) / w.num_bodies:.1f} per step per body)")
 
-    traffic = None
+    # figures of the committed rocprofv3 summaries of this workload, if any
+    traffic = profile_avg_us = None
     tpath = os.path.join(ROOT, "profiles", f"k_cg_iter_{args.config}_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+    kpath = os.path.join(ROOT, "profiles", f"k_cg_iter_{args.config}_kernel_trace.json")
+    if os.path.exists(kpath):
+        with open(kpath) as f:
+            profile_avg_us = json.load(f).get("avg_us_per_working_launch")
+    per_launch = {}
+    if w.num_bodies == 1 and total_iters > 0:
+        # one k_cg_iter launch is one CG iteration over the whole mesh
+        per_launch = {
+            "algorithmic_bytes_per_launch": BYTES_PER_TET_ITER * T + BYTES_PER_NODE_ITER * N,
+            "avg_launch_us": 1e3 * cg_ms / total_iters,
+            "avg_launch_note": "HIP events over the timed CG work divided by the CG iterations: one "
+                               "k_cg_iter launch plus its share of k_finish (position update), the "
+                               "early-exit launches of the last chunk and launch gaps",
+        }
 
     result = {
         "metric": "tet-steps/s at CG rel-res 1e-6, learned vs plain; CG HBM GB/s % of peak",
@@ -315,9 +330,13 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "CG kernels (k_cg_iter x iterations + k_finish)",
-            "algorithmic_bytes_per_launch": cg_bytes / args.steps,
-            "avg_launch_ms": cg_ms / max(1, nsteps),
+            "traffic_note": "HBM bytes per working k_cg_iter launch (rocprofv3 FETCH_SIZE / "
+                            "WRITE_SIZE scaled by a copy of known size)" if traffic else None,
+            "kernel": "k_cg_iter (one launch per CG iteration) + k_finish",
+            "algorithmic_bytes_per_step": cg_bytes / args.steps,
+            "cg_ms_per_step": cg_ms / max(1, nsteps),
+            **per_launch,
+            "profile_avg_launch_us": profile_avg_us,
         },
         "cpu_baseline": None,
     }
