@@ -338,8 +338,8 @@ static otfem_status blocks_build(const otfem_blocks_desc *d, double *coords, int
   if (!coords || !tets || !body_node_offsets || !body_tet_offsets)
     return fail(OTFEM_INVALID_ARG, "coords, tets and the body offsets are required");
   const bool has_load = d->load_axis >= 0;
-  if (d->fixed_axis > 2 || d->load_axis > 2 || (d->fixed_axis >= 0 && (d->fixed_side & ~1)) ||
-      (has_load && (d->load_side & ~1)))
+  if (d->fixed_axis < -1 || d->fixed_axis > 2 || d->load_axis < -1 || d->load_axis > 2 ||
+      (d->fixed_axis >= 0 && (d->fixed_side & ~1)) || (has_load && (d->load_side & ~1)))
     return fail(OTFEM_INVALID_ARG, "face axis must be -1, 0, 1 or 2 and side 0 or 1");
   if (f_ext && has_load && !d->traction) return fail(OTFEM_INVALID_ARG, "a load face needs a traction");
   if (d->jitter < 0.0 || !(d->jitter < 0.5))

@@ -164,3 +164,20 @@ def test_status_strings(fem):
     for code, text in names.items():
         assert lib.otfem_status_string(code) == text
     assert lib.otfem_status_string(99) == b"unknown status"
+
+
+def test_blocks_reject_bad_face_axis(fem):
+    dims = np.array([[2, 2, 2]], dtype=np.int32)
+    x = np.zeros(3 * 27)
+    d = fem.BlocksDesc()
+    d.num_bodies = 1
+    d.dims = dims.ctypes.data_as(C.POINTER(C.c_int32))
+    d.width = d.height = d.depth = 0.01
+    d.fixed_axis, d.load_axis = -2, -1
+    t = np.zeros(4 * 40, np.int32)
+    off = np.zeros(2, np.int64)
+    st = fem.load_library().otfem_blocks_build(
+        C.byref(d), x.ctypes.data_as(C.POINTER(C.c_double)), t.ctypes.data_as(C.POINTER(C.c_int32)),
+        None, None, off.ctypes.data_as(C.POINTER(C.c_int64)), off.ctypes.data_as(C.POINTER(C.c_int64)), None)
+    assert st == fem.INVALID_ARG
+    assert b"face axis" in fem.load_library().otfem_blocks_last_error()

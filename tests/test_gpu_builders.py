@@ -139,3 +139,13 @@ def test_workloads_same_on_device_and_host():
             assert np.array_equal(dv.body_node_offsets, hs.body_node_offsets)
             assert np.array_equal(dv.body_tet_offsets, hs.body_tet_offsets)
         assert np.abs(dv.forces - hs.forces).max() <= 1e-13 * np.abs(hs.forces).max()
+
+
+def test_random_source_failure_is_reported():
+    def bad(n):
+        raise ValueError("no random numbers today")
+    with pytest.raises(ValueError, match="no random numbers today"):
+        fem.build_blocks([(4, 4, 4)], H, jitter=0.2, min_volume=0.05 * H ** 3, draw=bad)
+    # and the library is usable afterwards
+    blk = fem.build_blocks([(2, 2, 2)], H)
+    assert blk.coords.shape == (27, 3)
