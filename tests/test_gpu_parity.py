@@ -394,3 +394,53 @@ def test_free_running_drift_relative_residual_mode():
         errs.append(rel_disp_error(s.state()[0], o.state()[0], w.coords))
     print("free-running drift: max", max(errs), "last", errs[-1], flush=True)
     assert max(errs) <= TOL
+
+
+@pytest.mark.parametrize("rotation_mode", [0, 1], ids=["gram_schmidt", "polar"])
+def test_rotations_of_random_deformations(rotation_mode):
+    """The pattern of the reference's unit_poloar_decomposition.cpp:26-60 on
+    the GPU path: 20,000 disconnected unit tets, each deformed by
+    A = R0 (S^T S + 0.1 I) with R0 a random proper rotation, rotated out by
+    one step with stored rotations.  Both modes: orthonormal, det +1, equal
+    to the oracle's (the reference's 1e-6).  Polar mode, the closed form:
+    the polar factor of A is R0, and R^T A is symmetric."""
+    from opentissue_amd import fem
+    import oracle
+    rng = np.random.default_rng(11)
+    T, h = 20000, 0.01
+    rest = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]]) * h
+    x0 = np.tile(rest, (T, 1))
+    tets = np.arange(4 * T, dtype=np.int32).reshape(T, 4)
+    Q, Rq = np.linalg.qr(rng.normal(size=(T, 3, 3)))
+    Q = Q * np.sign(np.diagonal(Rq, axis1=1, axis2=2))[:, None, :]
+    Q[np.linalg.det(Q) < 0, :, 0] *= -1
+    S = rng.normal(size=(T, 3, 3))
+    A = Q @ (S.transpose(0, 2, 1) @ S + 0.1 * np.eye(3))
+    x = (np.einsum("tij,kj->tki", A, rest) + rng.normal(size=(T, 1, 3)) * h).reshape(-1, 3)
+    s = fem.Solver(x0, tets)
+    s.init(5e5, 0.33, 1000.0)
+    s.set_state(x, np.zeros_like(x))
+    P = fem.reference_params()
+    P.rotation_mode, P.store_rotations = rotation_mode, 1
+    s.simulate(0.001, True, P)
+    R = s.rotations()
+    o = oracle.Oracle(x0, tets)
+    o.init(5e5, 0.33, 1000.0)
+    o.set_state(x, np.zeros_like(x))
+    op = oracle.reference_params()
+    op.rotation_mode = rotation_mode
+    o.simulate(0.001, True, op)
+    Ro = o.rotations()
+    orth = np.abs(np.einsum("tij,tkj->tik", R, R) - np.eye(3)).max()
+    vs_oracle = np.abs(R - Ro).max()
+    print("orthonormality", orth, "vs oracle", vs_oracle, flush=True)
+    assert orth < 1e-12
+    assert (np.linalg.det(R) > 0).all()
+    assert vs_oracle < 1e-6
+    if rotation_mode == 1:
+        RtA = R.transpose(0, 2, 1) @ A
+        closed = np.abs(R - Q).max()
+        sym = (np.abs(RtA - RtA.transpose(0, 2, 1)).max(axis=(1, 2)) / np.abs(A).max(axis=(1, 2))).max()
+        print("vs closed form", closed, "asymmetry of R^T A", sym, flush=True)
+        assert closed < 1e-9
+        assert sym < 1e-6
