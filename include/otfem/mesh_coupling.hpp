@@ -24,9 +24,11 @@
 #define OTFEM_MESH_COUPLING_HPP
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <thread>
 #include <vector>
 
 namespace otfem {
@@ -63,8 +65,10 @@ inline bool barycentric(const double *x1, const double *x2, const double *x3, co
 // For each of the n points pts[3n], the lowest-index tetrahedron (tets[4T]
 // over nodes[3N]) containing it within the tolerance, and the barycentric
 // coordinates of corners 1, 2, 3 (w123[3n]); tet[i] = -1 when none does.
+// The point queries run on `threads` threads (each point's result is its
+// own, so it does not depend on the thread count).
 inline void bind_points(int64_t N, const double *nodes, int64_t T, const int32_t *tets, int64_t n,
-                        const double *pts, double *w123, int32_t *tet) {
+                        const double *pts, double *w123, int32_t *tet, int threads = 1) {
   const double lower = -1e-4, upper = 1.0 + 1e-4;
   for (int64_t i = 0; i < n; ++i) {
     tet[i] = -1;
@@ -136,15 +140,15 @@ inline void bind_points(int64_t N, const double *nodes, int64_t T, const int32_t
         for (int64_t z = sp[2]; z <= sp[5]; ++z)
           list[(size_t)fill[(x * dim[1] + y) * dim[2] + z]++] = (int32_t)t;
   }
-  for (int64_t i = 0; i < n; ++i) {
+  auto query = [&](int64_t i) {
     const double *p = pts + 3 * i;
-    if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2])) continue;
+    if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2])) return;
     bool inside_box = true;
     for (int d = 0; d < 3; ++d) {
       const double pad = 1e-4 * (hi[d] - lo[d]) + 1e-12;
       inside_box = inside_box && p[d] >= lo[d] - pad && p[d] <= hi[d] + pad;
     }
-    if (!inside_box) continue;
+    if (!inside_box) return;
     const int64_t c = (cell_of(0, p[0]) * dim[1] + cell_of(1, p[1])) * dim[2] + cell_of(2, p[2]);
     for (int64_t k = start[c]; k < start[c + 1]; ++k) {
       const int32_t t = list[(size_t)k];
@@ -161,10 +165,22 @@ inline void bind_points(int64_t N, const double *nodes, int64_t T, const int32_t
         w123[3 * i + 1] = w[2];
         w123[3 * i + 2] = w[3];
         tet[i] = t;
-        break;
+        return;
       }
     }
-  }
+  };
+  const int64_t chunk = 4096;
+  const int64_t nchunks = (n + chunk - 1) / chunk;
+  const int64_t nt = std::max<int64_t>(1, std::min<int64_t>(threads, nchunks));
+  std::atomic<int64_t> next{0};
+  auto work = [&] {
+    for (int64_t k; (k = next.fetch_add(1, std::memory_order_relaxed)) < nchunks;)
+      for (int64_t i = k * chunk, e = std::min(n, i + chunk); i < e; ++i) query(i);
+  };
+  std::vector<std::thread> pool;
+  for (int64_t t = 1; t < nt; ++t) pool.emplace_back(work);
+  work();
+  for (auto &th : pool) th.join();
 }
 
 }  // namespace coupling

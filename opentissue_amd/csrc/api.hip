@@ -15,6 +15,7 @@
 #include <stdexcept>
 #include <tuple>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kernels.h"
@@ -1584,8 +1585,9 @@ static otfem_status otfem_surface_bind_body(otfem_handle h, int64_t num_vertices
                            h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   to_caller3(h->L, tmp, x.data());
+  const int threads = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
   coupling::bind_points(h->L.N, x.data(), h->L.T, h->tets.data(), num_vertices, vertices,
-                        barycentric, tet_index);
+                        barycentric, tet_index, threads);
   return OTFEM_OK;
 }
 

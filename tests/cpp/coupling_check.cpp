@@ -50,6 +50,26 @@ int main(int argc, char **argv) {
   std::vector<vec3> bary;
   std::vector<int> idx;
   OpenTissue::t4mesh::mesh_coupling::bind_surface(M, V, bary, idx);
+  {  // the array form gives the same binding on 1 and 4 threads
+    std::vector<double> nodes, pts(3 * n);
+    std::vector<int32_t> tets;
+    for (auto nd = V.node_begin(); nd != V.node_end(); ++nd)
+      for (int d = 0; d < 3; ++d) nodes.push_back(nd->m_coord(d));
+    for (auto t = V.tetrahedron_begin(); t != V.tetrahedron_end(); ++t) {
+      for (int k = 0; k < 4; ++k) tets.push_back((int32_t)t->node(k)->idx());
+    }
+    for (size_t i = 0; i < n; ++i)
+      for (int d = 0; d < 3; ++d) pts[3 * i + d] = M.v[i].m_coord(d);
+    std::vector<double> w1(3 * n), w4(3 * n);
+    std::vector<int32_t> t1(n), t4(n);
+    const int64_t N = (int64_t)nodes.size() / 3, T = (int64_t)tets.size() / 4;
+    otfem::coupling::bind_points(N, nodes.data(), T, tets.data(), (int64_t)n, pts.data(), w1.data(), t1.data(), 1);
+    otfem::coupling::bind_points(N, nodes.data(), T, tets.data(), (int64_t)n, pts.data(), w4.data(), t4.data(), 4);
+    if (w1 != w4 || t1 != t4) {
+      std::fprintf(stderr, "bind_points differs between 1 and 4 threads\n");
+      return 3;
+    }
+  }
   // affine motion x -> A x + t of every node
   const double A[3][3] = {{1.1, 0.2, -0.1}, {0.05, 0.9, 0.3}, {-0.2, 0.1, 1.3}};
   const double t[3] = {0.3, -0.7, 0.25};
