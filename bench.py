@@ -200,6 +200,8 @@ def main():
     ap.add_argument("--cap", type=int, default=5000, help="CG iteration cap (safety net)")
     ap.add_argument("--cg-block", type=int, default=0, choices=(0, 256, 512, 1024),
                     help="threads per tile in the CG kernels (0: autotune)")
+    ap.add_argument("--deterministic", action="store_true",
+                    help="per-node sums in slot order (same bits on every run)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -233,6 +235,8 @@ def main():
     log(f"otfem_create (tile layout) {time.perf_counter() - t_create:.2f} s")
     s.init_materials(w.young, w.poisson, w.density)
     s.set_external_forces(w.forces)
+    if args.deterministic:
+        s.set_deterministic(True)
     layout = s.layout_info()
     if args.cg_block or tuned_block:
         best_block = args.cg_block or tuned_block
@@ -318,6 +322,7 @@ def main():
             "tets": T, "nodes": N, "bodies": w.num_bodies,
             "tiles": layout["tiles"], "slots": layout["slots"], "cg_block": best_block,
             "tile_nodes": tile_nodes, "layout_autotuned": args.tile_nodes < 0,
+            "deterministic": bool(args.deterministic),
         },
         "cg_iterations_per_step": total_iters / args.steps / w.num_bodies,
         "cg_body_steps_unconverged": unconverged,

@@ -58,6 +58,10 @@ struct otfem_solver {
   double *rest64 = nullptr, *vd64 = nullptr;
   float4 *bt4 = nullptr;
   double4 *bt8 = nullptr;
+  // deterministic mode (otfem_set_deterministic)
+  bool ordered = false;
+  int32_t *ord_begin = nullptr, *ord_ent = nullptr;
+  double *ordered_terms = nullptr;
   double4 *tile_d = nullptr;  // [ntiles] D per unit volume and unit E of a tile's one nu
   bool one_nu_per_tile = false;  // slot records may use the folded form
   double *Re = nullptr, *plastic = nullptr, *plastic_param = nullptr;
@@ -174,7 +178,8 @@ void free_device(otfem_solver *h) {
   dfree(h->body); dfree(h->part_bb); dfree(h->part_rr);
   dfree(h->part4); dfree(h->done_count); dfree(h->iter_acc); dfree(h->unconv_acc);
   dfree(h->vw); dfree(h->hist1); dfree(h->hist2); dfree(h->kd); dfree(h->free_rank);
-  dfree(h->ws_bad);
+  dfree(h->ws_bad); dfree(h->act);
+  dfree(h->ord_begin); dfree(h->ord_ent); dfree(h->ordered_terms);
   drop_graphs(h);
   if (h->done_host) (void)hipHostFree(h->done_host);
   h->done_host = nullptr;
@@ -562,6 +567,10 @@ StepArgs make_args(otfem_solver *h, double dt, int warp, const otfem_params &P) 
   a.store64 = (h->lc.spmv_mode == SPMV_F64_STORE64) ? 1 : 0;
   a.tile_d = h->tile_d;
   a.folded = (h->one_nu_per_tile && h->lc.spmv_mode == SPMV_F64) ? 1 : 0;
+  a.ordered = h->ordered ? 1 : 0;
+  a.ord_begin = h->ord_begin;
+  a.ord_ent = h->ord_ent;
+  a.ordered_terms = h->ordered_terms;
   return a;
 }
 
@@ -1096,6 +1105,56 @@ otfem_status otfem_set_kernel_config(otfem_handle h, int cg_block, int spmv_mode
   h->lc.spmv_mode = spmv_mode;
   h->use_graphs = use_graphs != 0;
   return OTFEM_OK;
+}
+
+otfem_status otfem_set_deterministic(otfem_handle h, int on) {
+  if (!h) return OTFEM_INVALID_ARG;
+  if (!on) {
+    h->ordered = false;
+    dfree(h->ord_begin); dfree(h->ord_ent); dfree(h->ordered_terms);
+    return OTFEM_OK;
+  }
+  if (h->ordered) return OTFEM_OK;
+  otfem_status st = ensure_device(h);
+  if (st) return st;
+  return no_throw(h, [&]() -> otfem_status {
+    // per internal node, the (slot, corner) pairs it owns, in slot order:
+    // every node's terms are then added in the same order on every run
+    const Layout &L = h->L;
+    const int64_t N = L.N, S = (int64_t)L.slot_tet.size();
+    if (S >= (int64_t)1 << 29) return fail(h, OTFEM_INVALID_ARG, "too many slots for the deterministic mode");
+    std::vector<int32_t> begin(N + 1, 0);
+    for (const TileInfo &ti : L.tiles)
+      for (int64_t sl = ti.slot_begin; sl < ti.slot_begin + ti.n_slots; ++sl)
+        for (int c = 0; c < 4; ++c) {
+          const int32_t li = L.slot_conn[4 * sl + c];
+          if (li < ti.n_owned) ++begin[ti.node_begin + li + 1];
+        }
+    for (int64_t g = 0; g < N; ++g) begin[g + 1] += begin[g];
+    std::vector<int32_t> ent(std::max<int32_t>(begin[N], 1)), fill(begin.begin(), begin.end() - 1);
+    for (const TileInfo &ti : L.tiles)
+      for (int64_t sl = ti.slot_begin; sl < ti.slot_begin + ti.n_slots; ++sl)
+        for (int c = 0; c < 4; ++c) {
+          const int32_t li = L.slot_conn[4 * sl + c];
+          if (li < ti.n_owned) ent[fill[ti.node_begin + li]++] = (int32_t)(4 * sl + c);
+        }
+    hipError_t e = dalloc(&h->ord_begin, N + 1);
+    if (e == hipSuccess) e = dalloc(&h->ord_ent, ent.size());
+    if (e == hipSuccess) e = dalloc(&h->ordered_terms, 36 * (size_t)std::max<int64_t>(S, 1));
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(h->ord_begin, begin.data(), (N + 1) * sizeof(int32_t), hipMemcpyHostToDevice,
+                         h->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(h->ord_ent, ent.data(), ent.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                         h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+      dfree(h->ord_begin); dfree(h->ord_ent); dfree(h->ordered_terms);
+      return hip_fail(h, e, "deterministic mode buffers");
+    }
+    h->ordered = true;
+    return OTFEM_OK;
+  });
 }
 
 // Block-size sweep of otfem_autotune; *best_ms is the element + CG device

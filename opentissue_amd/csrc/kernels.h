@@ -84,6 +84,14 @@ struct StepArgs {
   int32_t plastic_on;
   int32_t store64;       // element kernel also writes bt8
   int32_t folded;        // slot records in folded form (every tile has one Poisson ratio)
+  // Ordered sums (deterministic mode): slots store their per-corner terms in
+  // `ordered_terms` (slot-major: one contiguous record per slot) and every
+  // owned node adds its terms in slot order, so results do not depend on
+  // wave scheduling.
+  int32_t ordered;
+  const int32_t *ord_begin;  // [N+1] internal node -> first entry
+  const int32_t *ord_ent;    // [4T] entries 4 * slot + corner, ascending per node
+  double *ordered_terms;     // [S][36] element terms (3 x [4][3]); CG uses [S][12]
 };
 
 struct LaunchConfig {

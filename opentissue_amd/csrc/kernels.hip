@@ -430,20 +430,57 @@ __global__ __launch_bounds__(kElemBlock) void k_element(StepArgs A) {
       }
     }
 
+    if (A.ordered) {
+      // deterministic mode: the slot's terms go to memory (one contiguous
+      // record) and are summed per node in slot order below
+      double *o = A.ordered_terms + 36 * (int64_t)s;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int li = cn[j];
-      if (li < no) {
+      for (int j = 0; j < 4; ++j)
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          atomicAdd(&sf[3 * li + d], fe[j][d]);
-          atomicAdd(&skv[3 * li + d], kv[j][d]);
-          if (A.plastic_on) atomicAdd(&sfp[3 * li + d], fpl[j][d]);
+          o[3 * j + d] = fe[j][d];
+          o[12 + 3 * j + d] = kv[j][d];
+          o[24 + 3 * j + d] = A.plastic_on ? fpl[j][d] : 0.0;
+        }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int li = cn[j];
+        if (li < no) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            atomicAdd(&sf[3 * li + d], fe[j][d]);
+            atomicAdd(&skv[3 * li + d], kv[j][d]);
+            if (A.plastic_on) atomicAdd(&sfp[3 * li + d], fpl[j][d]);
+          }
         }
       }
     }
   }
   __syncthreads();
+  if (A.ordered) {
+    for (int l = threadIdx.x; l < no; l += kElemBlock) {
+      const int64_t g = ti.node_begin + l;
+      double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      for (int e = A.ord_begin[g]; e < A.ord_begin[g + 1]; ++e) {
+        const int32_t code = A.ord_ent[e];
+        const double *o = A.ordered_terms + 36 * (int64_t)(code >> 2) + 3 * (code & 3);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          acc[d] += o[d];
+          acc[3 + d] += o[12 + d];
+          acc[6 + d] += o[24 + d];
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        sf[3 * l + d] = acc[d];
+        skv[3 * l + d] = acc[3 + d];
+        if (A.plastic_on) sfp[3 * l + d] = acc[6 + d];
+      }
+    }
+    __syncthreads();
+  }
 
   double bb = 0.0, rr = 0.0;
   for (int l = threadIdx.x; l < no; l += kElemBlock) {
@@ -606,11 +643,11 @@ __device__ __forceinline__ void unpack_raw<double, Folded>(const SlotRaw<Folded>
   D[2] = Dt[2];
 }
 
-// y_owned += (K' p)_e for one slot: G = sum_k (p_k - p_0) bt_k^T,
+// (K' p)_e for one slot, per corner: G = sum_k (p_k - p_0) bt_k^T,
 // sigma = D(G), f_k = sigma bt_k, f_0 = -(f_1 + f_2 + f_3).
-template <typename MT, typename AT>
-__device__ __forceinline__ void slot_spmv(ushort4 c, const MT bt[3][3], const MT D[3],
-                                          const double *sp, AT *sy, int no) {
+template <typename MT>
+__device__ __forceinline__ void slot_forces(ushort4 c, const MT bt[3][3], const MT D[3],
+                                            const double *sp, MT f[4][3]) {
   const double p0x = sp[3 * c.x], p0y = sp[3 * c.x + 1], p0z = sp[3 * c.x + 2];
   MT dp[3][3];
   const int cn[3] = {c.y, c.z, c.w};
@@ -628,7 +665,6 @@ __device__ __forceinline__ void slot_spmv(ushort4 c, const MT bt[3][3], const MT
       G[3 * r + q] = dp[0][r] * bt[0][q] + dp[1][r] * bt[1][q] + dp[2][r] * bt[2][q];
   MT Sg[9];
   stress(G, D[0], D[1], D[2], Sg);
-  MT f[4][3];
   f[0][0] = f[0][1] = f[0][2] = (MT)0;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -636,6 +672,14 @@ __device__ __forceinline__ void slot_spmv(ushort4 c, const MT bt[3][3], const MT
 #pragma unroll
     for (int d = 0; d < 3; ++d) f[0][d] -= f[k + 1][d];
   }
+}
+
+// y_owned += (K' p)_e for one slot (LDS atomics).
+template <typename MT, typename AT>
+__device__ __forceinline__ void slot_spmv(ushort4 c, const MT bt[3][3], const MT D[3],
+                                          const double *sp, AT *sy, int no) {
+  MT f[4][3];
+  slot_forces<MT>(c, bt, D, sp, f);
   const int cc[4] = {c.x, c.y, c.z, c.w};
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -645,6 +689,32 @@ __device__ __forceinline__ void slot_spmv(ushort4 c, const MT bt[3][3], const MT
     }
   }
 }
+
+// Deterministic mode: the slot's four corner terms to out[12] (one
+// contiguous record in memory); each owned node adds its terms in slot order
+// after the slot loop.
+template <typename MT>
+__device__ __forceinline__ void slot_spmv_store(ushort4 c, const MT bt[3][3], const MT D[3],
+                                                const double *sp, double *out) {
+  MT f[4][3];
+  slot_forces<MT>(c, bt, D, sp, f);
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out[3 * j + d] = (double)f[j][d];
+}
+
+// Accumulator tag of the deterministic mode (K'p summed per node in slot
+// order, in FP64); AccOf gives the LDS type of the node sums.
+struct Ordered {};
+template <typename AT>
+struct AccOf {
+  using type = AT;
+};
+template <>
+struct AccOf<Ordered> {
+  using type = double;
+};
 
 }  // namespace
 
@@ -796,6 +866,8 @@ void k_cg_iter(StepArgs A, int it) {
   __shared__ double red[4 * (BLOCK / 64 + 1)];
   constexpr int NB = CgShape<BLOCK>::kNodes;
   constexpr int U = sizeof(ST) == 8 ? (CgShape<BLOCK>::kSlots + 1) / 2 : CgShape<BLOCK>::kSlots;
+  constexpr bool kOrdered = std::is_same<AT, Ordered>::value;
+  using Acc = typename AccOf<AT>::type;
   int tile;
   if (A.compact) {
     // workgroup -> active tile; the remap keeps neighbouring tiles on one XCD
@@ -900,7 +972,7 @@ void k_cg_iter(StepArgs A, int it) {
   double *sp = lds;                 // p of every tile node
   double *sr = sp + 3 * nn;         // r of owned nodes
   double *sm = sr + 3 * no;         // mass of owned nodes, -1 if fixed
-  AT *sy = reinterpret_cast<AT *>(sm + no);  // K'p of owned nodes
+  Acc *sy = reinterpret_cast<Acc *>(sm + no);  // K'p of owned nodes
 
   // prologue: finish iteration it-1 and form the new direction, NB nodes per
   // thread at a time (the first batch was loaded above)
@@ -908,7 +980,7 @@ void k_cg_iter(StepArgs A, int it) {
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
       const int l = base + (int)threadIdx.x + k * BLOCK;
-      if (l < nn) update_node<AT>(A, it, stop, alpha, beta, l, no, nd[k], Rn, Pn, sp, sr, sy, sm);
+      if (l < nn) update_node<Acc>(A, it, stop, alpha, beta, l, no, nd[k], Rn, Pn, sp, sr, sy, sm);
     }
     base += NB * BLOCK;
     if (base >= nn) break;
@@ -928,7 +1000,10 @@ void k_cg_iter(StepArgs A, int it) {
         if (s + u * BLOCK < send) {
           MT bt[3][3], D[3];
           unpack_raw<MT, ST>(ra[u], Dt, bt, D);
-          slot_spmv<MT, AT>(ra[u].c, bt, D, sp, sy, no);
+          if constexpr (kOrdered)
+            slot_spmv_store<MT>(ra[u].c, bt, D, sp, A.ordered_terms + 12 * (int64_t)(s + u * BLOCK));
+          else
+            slot_spmv<MT, AT>(ra[u].c, bt, D, sp, sy, no);
         }
       }
       if (s + U * BLOCK >= send) break;
@@ -940,7 +1015,11 @@ void k_cg_iter(StepArgs A, int it) {
         if (s + (U + u) * BLOCK < send) {
           MT bt[3][3], D[3];
           unpack_raw<MT, ST>(rb[u], Dt, bt, D);
-          slot_spmv<MT, AT>(rb[u].c, bt, D, sp, sy, no);
+          if constexpr (kOrdered)
+            slot_spmv_store<MT>(rb[u].c, bt, D, sp,
+                                A.ordered_terms + 12 * (int64_t)(s + (U + u) * BLOCK));
+          else
+            slot_spmv<MT, AT>(rb[u].c, bt, D, sp, sy, no);
         }
       }
     }
@@ -950,7 +1029,23 @@ void k_cg_iter(StepArgs A, int it) {
   double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
   for (int l = threadIdx.x; l < no; l += BLOCK) {
     const double m = sm[l];
-    if (m >= 0.0) node_partials<AT>(A, ti.node_begin + l, l, m, sp, sr, sy, Yn, acc);
+    if constexpr (kOrdered) {
+      // this node's K'p: its slot terms added in slot order (the thread that
+      // writes sy[l] here is the one that reads it below)
+      const int64_t g = ti.node_begin + l;
+      double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+      for (int e = A.ord_begin[g]; e < A.ord_begin[g + 1]; ++e) {
+        const int32_t code = A.ord_ent[e];
+        const double *o = A.ordered_terms + 12 * (int64_t)(code >> 2) + 3 * (code & 3);
+        y0 += o[0];
+        y1 += o[1];
+        y2 += o[2];
+      }
+      sy[3 * l] = y0;
+      sy[3 * l + 1] = y1;
+      sy[3 * l + 2] = y2;
+    }
+    if (m >= 0.0) node_partials<Acc>(A, ti.node_begin + l, l, m, sp, sr, sy, Yn, acc);
   }
   const double4 tot = block_sum4<BLOCK>(acc, red);
   if (threadIdx.x == 0) reinterpret_cast<double4 *>(A.part4 + cur * 4 * (int64_t)nt)[tile] = tot;
@@ -1063,6 +1158,12 @@ __global__ __launch_bounds__(kCompactBlock) void k_compact(StepArgs A) {
 template <int BLOCK>
 hipError_t enqueue_mode(const StepArgs &a, int mode, size_t lds_cg, hipStream_t s, int it0,
                         int it1, int g) {
+  if (a.ordered) {  // deterministic mode: FP64 arithmetic, ordered node sums
+    if (mode == SPMV_F64_STORE64)
+      return enqueue_iterations<BLOCK, double, Ordered, double>(a, lds_cg, s, it0, it1, g);
+    if (a.folded) return enqueue_iterations<BLOCK, double, Ordered, Folded>(a, lds_cg, s, it0, it1, g);
+    return enqueue_iterations<BLOCK, double, Ordered, float>(a, lds_cg, s, it0, it1, g);
+  }
   switch (mode) {
     case SPMV_F32: return enqueue_iterations<BLOCK, float, float, float>(a, lds_cg, s, it0, it1, g);
     case SPMV_F32_ACC64:

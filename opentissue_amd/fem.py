@@ -142,6 +142,7 @@ SIGNATURES = {
     "otfem_iteration_totals": (C.c_int, [_H, C.POINTER(C.c_int64), C.c_int]),
     "otfem_unconverged_totals": (C.c_int, [_H, C.POINTER(C.c_int64), C.c_int]),
     "otfem_set_kernel_config": (C.c_int, [_H, C.c_int, C.c_int, C.c_int]),
+    "otfem_set_deterministic": (C.c_int, [_H, C.c_int]),
     "otfem_autotune": (C.c_int, [_H, C.c_double, C.c_int, C.POINTER(Params), C.c_int,
                                  C.POINTER(C.c_int)]),
     "otfem_body_stats": (C.c_int, [_H, C.POINTER(C.c_int), _dp]),
@@ -429,6 +430,11 @@ class Solver:
         """Threads per tile in the CG kernels (256, 512, 1024) and the SpMV
         precision (SPMV_*; the library's default is SPMV_F64)."""
         self._check(self.lib.otfem_set_kernel_config(self.h, cg_block, spmv_mode, 1 if use_graphs else 0))
+
+    def set_deterministic(self, on=True):
+        """Per-node sums in a fixed order: the same bits on every run
+        (``otfem_set_deterministic``)."""
+        self._check(self.lib.otfem_set_deterministic(self.h, 1 if on else 0))
 
     def autotune(self, dt, warp=True, params=None, steps=2):
         """Pick the fastest CG block size for this mesh (state is restored)."""

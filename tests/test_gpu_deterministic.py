@@ -1,0 +1,154 @@
+"""Deterministic mode (otfem_set_deterministic): every per-node sum of the step
+is added in slot order, so the same step from the same state gives the same
+bits on every run -- single and batched meshes, plasticity, the learned start,
+every CG block size and record form, and across a checkpoint.  Its results
+meet the same 1e-6 per-step bar against the FP64 oracle, and stay within
+rounding of the default (LDS-atomic) mode."""
+import numpy as np
+import pytest
+
+from conftest import rel_disp_error
+
+pytestmark = pytest.mark.gpu
+
+H = 0.01
+
+
+def _cube(n=8, seed=1, plastic=False):
+    from opentissue_amd import fem, mesh
+    c0, t = mesh.generate_blocks(n, n + 2, n - 1, H, H, H)
+    c = mesh.jitter_interior(c0, t, H, 0.2, seed)
+    fx = mesh.face_nodes(c, 0, 0).astype(np.uint8)
+    f = mesh.traction_forces(c, t, 0, 1, np.array([0.0, 1e3, -2e3]))
+    f[fx == 1] = 0.0
+    E = np.exp(np.random.default_rng(seed).uniform(np.log(1e5), np.log(1e6), len(t)))
+    kw = dict(c_yield=0.0, c_creep=0.5, c_max=0.1) if plastic else {}
+    return c, t, fx, f, E, kw
+
+
+def _run(c, t, fx, f, E, kw, steps, P, deterministic=True, block=0, spmv=None, bodies=None):
+    from opentissue_amd import fem
+    s = fem.Solver(c, t, fx, **(bodies or {}))
+    s.init_materials(E, 0.3, 1000.0, **kw)
+    s.set_external_forces(f)
+    if block or spmv is not None:
+        s.set_kernel_config(cg_block=block or 256, spmv_mode=fem.SPMV_F64 if spmv is None else spmv)
+    if deterministic:
+        s.set_deterministic(True)
+    for _ in range(steps):
+        s.simulate(0.005, True, P)
+    x, v = s.state()
+    pl = s.plastic()
+    s.close()
+    return x, v, pl
+
+
+@pytest.mark.parametrize("block,spmv", [(0, None), (512, None), (1024, None), (256, "store64")])
+def test_same_bits_on_every_run(block, spmv):
+    from opentissue_amd import fem
+    c, t, fx, f, E, kw = _cube()
+    mode = None if spmv is None else fem.SPMV_F64_STORE64
+    P = fem.config_params(cap=3000)
+    a = _run(c, t, fx, f, E, kw, 5, P, block=block, spmv=mode)
+    b = _run(c, t, fx, f, E, kw, 5, P, block=block, spmv=mode)
+    print("case", block, spmv, "max |x - x0|", np.abs(a[0] - c).max(), flush=True)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_batched_plastic_bodies_same_bits_and_close_to_default():
+    from opentissue_amd import fem, mesh
+    parts = [mesh.generate_blocks(*d, H, H, H) for d in [(3, 4, 5), (6, 2, 3), (2, 2, 2)]]
+    c, t, no, to = mesh.pack_bodies(parts)
+    fx = np.zeros(len(c), np.uint8)
+    f = np.zeros_like(c)
+    for b in range(3):
+        cb, tb = c[no[b]:no[b + 1]], t[to[b]:to[b + 1]] - no[b]
+        fx[no[b]:no[b + 1]] = mesh.face_nodes(cb, 0, 0)
+        f[no[b]:no[b + 1]] = mesh.traction_forces(cb, tb, 0, 1, np.array([0.0, 0.0, -4e3]))
+    f[fx == 1] = 0.0
+    E = np.repeat([2e5, 5e5, 8e5], np.diff(to))
+    kw = dict(c_yield=0.0, c_creep=0.5, c_max=0.1)
+    bodies = dict(body_node_offsets=no, body_tet_offsets=to)
+    P = fem.config_params(cap=3000)
+    a = _run(c, t, fx, f, E, kw, 4, P, bodies=bodies)
+    b = _run(c, t, fx, f, E, kw, 4, P, bodies=bodies)
+    assert np.abs(a[2]).max() > 0  # plasticity is active
+    assert all(np.array_equal(p, q) for p, q in zip(a, b))
+    d = _run(c, t, fx, f, E, kw, 4, P, deterministic=False, bodies=bodies)
+    err = rel_disp_error(a[0], d[0], c)
+    print("deterministic vs default", err, flush=True)
+    assert err <= 1e-6
+
+
+@pytest.mark.parametrize("mode", ["ref", "cfg"])
+def test_per_step_parity_with_oracle(mode):
+    from opentissue_amd import fem
+    import oracle
+    c, t, fx, f, E, kw = _cube(6, seed=4)
+    s = fem.Solver(c, t, fx)
+    s.init_materials(E, 0.3, 1000.0)
+    s.set_external_forces(f)
+    s.set_deterministic(True)
+    o = oracle.Oracle(c, t, fx)
+    o.init_materials(E, 0.3, 1000.0)
+    o.set_external_forces(f)
+    gp = fem.reference_params() if mode == "ref" else fem.config_params(cap=3000)
+    op = oracle.reference_params() if mode == "ref" else oracle.config_params(cap=3000)
+    errs = []
+    for _ in range(4):
+        xo, vo = o.state()
+        s.set_state(xo, vo)
+        s.simulate(0.005, True, gp)
+        o.simulate(0.005, True, op)
+        errs.append(rel_disp_error(s.state()[0], o.state()[0], c))
+    print("per-step errors", errs, flush=True)
+    assert max(errs) <= 1e-6
+
+
+def test_learned_start_and_checkpoint_resume_same_bits(tmp_path):
+    from opentissue_amd import fem
+    c, t, fx, f, E, kw = _cube(7, seed=2, plastic=True)
+
+    def solver():
+        s = fem.Solver(c, t, fx)
+        s.init_materials(E, 0.3, 1000.0, **kw)
+        s.set_external_forces(f)
+        s.set_deterministic(True)
+        s.set_warmstart_model(fem.Model(seed=9))
+        return s
+
+    P = fem.config_params(cap=3000)
+    L = fem.config_params(cap=3000)
+    L.warm_start = fem.WARM_LEARNED
+    a = solver()
+    for _ in range(3):
+        a.simulate(0.005, True, P)
+    path = str(tmp_path / "det_state.bin")
+    a.save_state(path)
+    for _ in range(3):
+        a.simulate(0.005, True, L)
+    b = solver()
+    b.load_state(path)
+    for _ in range(3):
+        b.simulate(0.005, True, L)
+    xa, va = a.state()
+    xb, vb = b.state()
+    assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+    assert np.array_equal(a.plastic(), b.plastic())
+    a.close()
+    b.close()
+
+
+def test_switching_off_returns_to_the_default_mode():
+    from opentissue_amd import fem
+    c, t, fx, f, E, kw = _cube(5)
+    s = fem.Solver(c, t, fx)
+    s.init_materials(E, 0.3, 1000.0)
+    s.set_external_forces(f)
+    s.set_deterministic(True)
+    s.simulate(0.005, True, fem.config_params())
+    s.set_deterministic(False)
+    s.simulate(0.005, True, fem.config_params())
+    x, _ = s.state()
+    assert np.isfinite(x).all() and np.abs(x - c).max() > 0
+    s.close()
