@@ -3,7 +3,7 @@
 Error metric (per step, from a shared state): after setting the GPU solver to
 the oracle's (x, v) and taking one step on both,
     e = max_i |x_gpu - x_oracle| / max_i |x_oracle - x0|.
-The bar is 1e-6 (north-star tolerance for this floating-point path).
+The bar is 1e-6, the per-step tolerance this floating-point path is held to.
 """
 import numpy as np
 import pytest
