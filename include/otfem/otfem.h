@@ -91,7 +91,9 @@ typedef struct {
   int num_bodies;              /* 0 or 1: one body; >1: CSR of bodies below */
   const int64_t *body_node_offsets; /* [B+1]: body b owns nodes [o[b], o[b+1]) */
   const int64_t *body_tet_offsets;  /* [B+1]: body b owns tets  [o[b], o[b+1]) */
-  int target_tile_nodes;       /* owned nodes per device tile, 0 = default */
+  int target_tile_nodes;       /* owned nodes per device tile, 0 = default: ~512, or for
+                                  one body a multiple of 256 tiles, or up to 248 smaller
+                                  tiles when the body is too small for that */
 } otfem_mesh_desc;
 
 typedef struct otfem_solver *otfem_handle;
@@ -185,7 +187,8 @@ otfem_status otfem_autotune(otfem_handle h, double dt, int use_stiffness_warping
  * NULL), runs the otfem_autotune sweep on it, and returns the tile target and
  * CG block size of the fastest (element + CG device time of `steps` steps;
  * *best_ms may be NULL).  candidates NULL / num_candidates 0: for one body
- * the default tiling and 256, 768 and 1,024 tiles, for several bodies 256,
+ * the default tiling, 512 owned nodes per tile, and 120, 256, 768 and 1,024
+ * tiles (where tiles keep at least 16 / 64 nodes), for several bodies 256,
  * 512 and 1,024 owned nodes per tile.  Candidates that cannot be built (a
  * tile beyond the LDS budget) are skipped.  Create the real solver with
  * desc.target_tile_nodes = *best_tile_nodes and apply the block size with

@@ -1255,10 +1255,12 @@ static otfem_status autotune_layout_body(const otfem_mesh_desc *desc, const doub
   if (num_candidates > 0) {
     cand.assign(candidates, candidates + num_candidates);
   } else if (desc->num_bodies <= 1) {
-    // one body: the default (a CU-multiple tile count near 512 owned nodes)
-    // and 256, 768 and 1,024 tiles
+    // one body: the default tiling, 512 owned nodes per tile, and 120, 256,
+    // 768 and 1,024 tiles where the body has at least 16 (120 tiles) or 64
+    // nodes per tile
     const int64_t N = desc->num_nodes;
-    cand = {0};
+    cand = {0, 512};
+    if (N >= 16 * 120) cand.push_back((int32_t)((N + 119) / 120));
     for (int64_t tiles : {256, 768, 1024})
       if (N >= 64 * tiles) cand.push_back((int32_t)((N + tiles - 1) / tiles));
   } else {

@@ -158,7 +158,14 @@ def test_slot_record_forms(materials):
 def test_irregular_mesh_read_from_xml(mode):
     """An irregular (TetGen-made) mesh entering through the T4MESH XML reader:
     the reference's fixture demos/data/xml/blocky.xml (438 nodes, 1,831 tets),
-    fixed at its lowest y, under gravity along -z."""
+    fixed at its lowest y, under gravity along -z.
+
+    The relative-residual mode solves to 1e-8 here.  At 1e-6 this mesh's CG
+    converges in ~45 iterations, so two correct runs whose rounding differs
+    can stop one iteration apart (measured: 47 against 48 on step 1), and one
+    iteration near the stop moves the iterate by ~1e-6 of the first step's
+    small displacement (1.1e-6 measured; 1e-8 where the counts agree).  The
+    tighter stop compares the arithmetic, not the stopping point."""
     import os
     from opentissue_amd import fem, mesh_io
     import oracle
@@ -166,8 +173,8 @@ def test_irregular_mesh_read_from_xml(mode):
     fx = (c[:, 1] < c[:, 1].min() + 1e-9).astype(np.uint8)
     assert 0 < fx.sum() < len(fx)
     s, o = _pair(c, t, fx)
-    gp = fem.reference_params() if mode == "ref" else fem.config_params(cap=3000)
-    op = oracle.reference_params() if mode == "ref" else oracle.config_params(cap=3000)
+    gp = fem.reference_params() if mode == "ref" else fem.config_params(tol=1e-8, cap=3000)
+    op = oracle.reference_params() if mode == "ref" else oracle.config_params(tol=1e-8, cap=3000)
     errs, iters = _per_step_errors(s, o, c, 5, 0.01, gp, op)
     print(mode, errs, iters)
     assert errs.max() <= TOL, errs
