@@ -188,9 +188,9 @@ otfem_status otfem_autotune(otfem_handle h, double dt, int use_stiffness_warping
  * CG block size of the fastest (element + CG device time of `steps` steps;
  * *best_ms may be NULL).  candidates NULL / num_candidates 0: for one body
  * the default tiling, 512 owned nodes per tile, and 120, 256, 768 and 1,024
- * tiles (where tiles keep at least 16 / 64 nodes), for several bodies 256,
- * 512 and 1,024 owned nodes per tile.  Candidates that cannot be built (a
- * tile beyond the LDS budget) are skipped.  Create the real solver with
+ * tiles (where that leaves 16 / 64 nodes per tile on average); for several
+ * bodies 256, 512 and 1,024 owned nodes per tile.  Candidates that cannot be
+ * built (a tile beyond the LDS budget) are skipped.  Create the real solver with
  * desc.target_tile_nodes = *best_tile_nodes and apply the block size with
  * otfem_set_kernel_config.  No reference counterpart (tuning of this
  * implementation). */

@@ -1256,8 +1256,8 @@ static otfem_status autotune_layout_body(const otfem_mesh_desc *desc, const doub
     cand.assign(candidates, candidates + num_candidates);
   } else if (desc->num_bodies <= 1) {
     // one body: the default tiling, 512 owned nodes per tile, and 120, 256,
-    // 768 and 1,024 tiles where the body has at least 16 (120 tiles) or 64
-    // nodes per tile
+    // 768 and 1,024 tiles where that leaves 16 (120 tiles) or 64 owned nodes
+    // per tile on average
     const int64_t N = desc->num_nodes;
     cand = {0, 512};
     if (N >= 16 * 120) cand.push_back((int32_t)((N + 119) / 120));

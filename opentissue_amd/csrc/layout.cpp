@@ -246,8 +246,8 @@ std::string build_layout(int64_t N, int64_t T, const double *coords, const int32
           // A body too small to give every CU a 512-node tile is
           // latency-bound: more, smaller tiles shorten an iteration although
           // more tets are copied into several tiles.  About one tile per CU,
-          // up to 248 so that none waits for a second round, and at least 16
-          // owned nodes each (measured, 3 steps: a 4,961-node cantilever
+          // up to 248 so that none waits for a second round, and 16 owned
+          // nodes per tile on average (measured, 3 steps: a 4,961-node cantilever
           // 6.6 ms on 10 tiles, 3.7 ms on 249; a 30^3 cube 14.4 ms on 59
           // tiles, 9.4 ms on 233, 11.9 ms on 257).
           parts = std::max<int64_t>(1, std::min<int64_t>(248, (n1 - n0) / 16));

@@ -1,7 +1,8 @@
 // Host check of the tile layout (opentissue_amd/csrc/layout.cpp), built with
 // g++ from the layout source alone:
 //   * the layout is the same for 1, 3 and 8 host threads (batched bodies and
-//     one large body with the CU-multiple default tiling);
+//     one large body with the CU-multiple default tiling), and a small body
+//     gets the small-body default (min(248, N / 16) tiles);
 //   * structure: node ids form a permutation, tiles never mix bodies, every
 //     tet has exactly one primary slot and one slot per distinct owning tile
 //     of its corners, local corner indices resolve to the right nodes, and
@@ -9,6 +10,7 @@
 //     repeats no owned corner;
 //   * errors: an out-of-range index is reported before a tet that reaches
 //     into another body.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -169,6 +171,26 @@ int main() {
   CHECK(otfem::build_layout(N1, T1, x1.data(), t1.data(), 1, nullptr, nullptr, 0, 16384, one, 8).empty());
   CHECK(same(one_ref, one));
   structure(one, t1);
+  // a small single body: min(248, N / 16) tiles (16 owned nodes on average;
+  // the bisection balances slot work, so interior tiles hold fewer)
+  {
+    std::vector<double> xs;
+    std::vector<int32_t> ts;
+    lattice(10, 10, 10, 0.01, seed, xs, ts);
+    const int64_t Ns = (int64_t)xs.size() / 3, Ts = (int64_t)ts.size() / 4;
+    Layout small;
+    CHECK(otfem::build_layout(Ns, Ts, xs.data(), ts.data(), 1, nullptr, nullptr, 0, 16384, small, 8).empty());
+    CHECK((int64_t)small.tiles.size() == std::min<int64_t>(248, Ns / 16));
+    int32_t lo = 1 << 30, hi = 0;
+    for (const TileInfo &ti : small.tiles) {
+      CHECK(ti.n_owned >= 1);
+      lo = std::min(lo, ti.n_owned);
+      hi = std::max(hi, ti.n_owned);
+    }
+    std::printf("small body: %lld nodes, %zu tiles of %d..%d owned nodes\n", (long long)Ns,
+                small.tiles.size(), lo, hi);
+    structure(small, ts);
+  }
   // errors: out of range wins over a cross-body index wherever they are
   {
     std::vector<int32_t> bad = t;

@@ -80,7 +80,7 @@ def test_layout_counts(fem, dims, tile):
     s = fem.Solver(c, t, target_tile_nodes=tile)
     info = s.layout_info()
     N, T = len(c), len(t)
-    # default for one body this small: up to 248 tiles of at least 16 nodes
+    # default for one body this small: min(248, N / 16) tiles
     want = -(-N // tile) if tile else max(1, min(248, N // 16))
     assert info["tiles"] == want
     # every tet has 1..4 copies (one per distinct owning tile of its corners)
