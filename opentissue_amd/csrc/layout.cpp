@@ -193,6 +193,23 @@ std::string build_layout(int64_t N, int64_t T, const double *coords, const int32
     if (bad.load() == 2) return "a tetrahedron references a node of another body";
   }
 
+  // A batch whose 512-node tiling leaves CUs idle (fewer than 248 tiles in
+  // all) is latency-bound like a small single body: its bodies get smaller
+  // tiles, the largest node target (16 at least) that keeps the batch within
+  // 248 tiles.
+  if (auto_tiles && nb > 1) {
+    auto tiles_at = [&](int64_t target) {
+      int64_t total = 0;
+      for (int64_t b = 0; b < nb; ++b) total += (bn[b + 1] - bn[b] + target - 1) / target;
+      return total;
+    };
+    if (tiles_at(512) < 248) {
+      int64_t target = std::max<int64_t>(16, (N + 247) / 248);
+      while (target < 512 && tiles_at(target) > 248) target += target / 8 + 1;
+      target_tile_nodes = (int)std::min<int64_t>(target, 512);
+    }
+  }
+
   // 1. Partition nodes of each body into tiles (internal ids = tile-major).
   // A tile's CG work is mostly its slot count.  Nodes inside a body touch
   // more tets than nodes on its surface, so tiles of equal node count hold

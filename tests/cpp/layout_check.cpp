@@ -191,6 +191,23 @@ int main() {
                 small.tiles.size(), lo, hi);
     structure(small, ts);
   }
+  // a small batch: smaller tiles, at most 248 of them
+  {
+    std::vector<double> xb;
+    std::vector<int32_t> tb;
+    std::vector<int64_t> nob{0}, tob{0};
+    for (int b = 0; b < 8; ++b) {
+      lattice(6 + b % 3, 7, 5, 0.01, seed, xb, tb);
+      nob.push_back((int64_t)xb.size() / 3);
+      tob.push_back((int64_t)tb.size() / 4);
+    }
+    Layout lb;
+    CHECK(otfem::build_layout(nob.back(), tob.back(), xb.data(), tb.data(), 8, nob.data(), tob.data(),
+                              0, 16384, lb, 8).empty());
+    CHECK(lb.tiles.size() > 100 && lb.tiles.size() <= 248);
+    std::printf("small batch: %lld nodes, %zu tiles\n", (long long)nob.back(), lb.tiles.size());
+    structure(lb, tb);
+  }
   // errors: out of range wins over a cross-body index wherever they are
   {
     std::vector<int32_t> bad = t;
