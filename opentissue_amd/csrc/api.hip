@@ -64,7 +64,7 @@ struct otfem_solver {
   double *ordered_terms = nullptr;
   double4 *tile_d = nullptr;  // [ntiles] D per unit volume and unit E of a tile's one nu
   bool one_nu_per_tile = false;  // slot records may use the folded form
-  double *Re = nullptr, *plastic = nullptr, *plastic_param = nullptr;
+  double *Re = nullptr, *plastic = nullptr, *plastic_next = nullptr, *plastic_param = nullptr;
   BodyState *body = nullptr;
   double *part_bb = nullptr, *part_rr = nullptr, *part4 = nullptr;
   int *done_count = nullptr;  // [2]: stopped bodies, active tiles
@@ -174,7 +174,7 @@ void free_device(otfem_solver *h) {
   dfree(h->rhs); dfree(h->r); dfree(h->p); dfree(h->y); dfree(h->fixed);
   dfree(h->tiles); dfree(h->halo); dfree(h->conn); dfree(h->slot_tet);
   dfree(h->rest64); dfree(h->vd64); dfree(h->bt4); dfree(h->bt8); dfree(h->tile_d);
-  dfree(h->Re); dfree(h->plastic); dfree(h->plastic_param);
+  dfree(h->Re); dfree(h->plastic); dfree(h->plastic_next); dfree(h->plastic_param);
   dfree(h->body); dfree(h->part_bb); dfree(h->part_rr);
   dfree(h->part4); dfree(h->done_count); dfree(h->iter_acc); dfree(h->unconv_acc);
   dfree(h->vw); dfree(h->hist1); dfree(h->hist2); dfree(h->kd); dfree(h->free_rank);
@@ -297,6 +297,7 @@ otfem_status ensure_device(otfem_solver *h) {
   HIPCHK(h, dalloc(&h->tile_d, nt));
   HIPCHK(h, dalloc(&h->Re, 9 * T));
   HIPCHK(h, dalloc(&h->plastic, 6 * T));
+  HIPCHK(h, dalloc(&h->plastic_next, 6 * T));
   HIPCHK(h, dalloc(&h->plastic_param, 3 * T));
   HIPCHK(h, dalloc(&h->body, B));
   HIPCHK(h, dalloc(&h->part_bb, nt));
@@ -541,7 +542,8 @@ StepArgs make_args(otfem_solver *h, double dt, int warp, const otfem_params &P) 
   a.rhs = h->rhs; a.r = h->r; a.p = h->p; a.y = h->y;
   a.tiles = h->tiles; a.halo = h->halo; a.conn = h->conn; a.slot_tet = h->slot_tet;
   a.rest64 = h->rest64; a.vd64 = h->vd64; a.bt4 = h->bt4; a.bt8 = h->bt8;
-  a.Re = h->Re; a.plastic = h->plastic; a.plastic_param = h->plastic_param;
+  a.Re = h->Re; a.plastic = h->plastic; a.plastic_next = h->plastic_next;
+  a.plastic_param = h->plastic_param;
   a.body = h->body;
   a.part_bb = h->part_bb; a.part_rr = h->part_rr; a.part4 = h->part4;
   a.done_count = h->done_count; a.iter_acc = h->iter_acc; a.unconv_acc = h->unconv_acc;
@@ -954,6 +956,8 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
     std::memcpy(key.data(), &a, sizeof(StepArgs));
     std::memcpy(key.data() + sizeof(StepArgs), &h->lc, sizeof(LaunchConfig));
     if (key != h->graph_key || !h->g_elem || !h->g_fin) {
+      // earlier steps may still be running the graphs about to be destroyed
+      if (h->g_elem || h->g_fin || !h->g_chunks.empty()) HIPCHK(h, hipStreamSynchronize(s));
       drop_graphs(h);
       otfem_status cs = capture([&](hipStream_t cs_) { return enqueue_element(a, h->lds_elem, cs_); },
                                 &h->g_elem, "enqueue_element");
@@ -1110,6 +1114,8 @@ otfem_status otfem_set_kernel_config(otfem_handle h, int cg_block, int spmv_mode
 otfem_status otfem_set_deterministic(otfem_handle h, int on) {
   if (!h) return OTFEM_INVALID_ARG;
   if (!on) {
+    // steps enqueued without stats may still use the buffers
+    if (h->device_ready) HIPCHK(h, hipStreamSynchronize(h->stream));
     h->ordered = false;
     dfree(h->ord_begin); dfree(h->ord_ent); dfree(h->ordered_terms);
     return OTFEM_OK;

@@ -50,7 +50,9 @@ struct StepArgs {
   double4 *bt8;          // [3][S]  same in FP64 (only with SPMV_F64_STORE64)
   // per tet (caller order)
   double *Re;            // [T][9]
-  double *plastic;       // [T][6]
+  double *plastic;       // [T][6]  plastic strain at the start of the step (read only in k_element)
+  double *plastic_next;  // [T][6]  the step's updated strain, written by each tet's primary
+                         //         copy and committed to `plastic` after k_element
   const double *plastic_param;  // [T][3] yield, creep, max
   const double4 *tile_d; // [ntiles] (D0, D1, D2, 0) per unit volume and unit E of a tile's one nu
   // bodies / reductions (no atomics except done_count)

@@ -405,9 +405,12 @@ __global__ __launch_bounds__(kElemBlock) void k_element(StepArgs A) {
 #pragma unroll
         for (int i = 0; i < 6; ++i) pl[i] *= sc;
       }
+      // every copy of the tet reads the strain of the start of the step and
+      // updates it the same way; the primary copy stores the result apart, so
+      // no copy can read a strain another tile has already updated
       if (primary) {
 #pragma unroll
-        for (int i = 0; i < 6; ++i) A.plastic[6 * tet + i] = pl[i];
+        for (int i = 0; i < 6; ++i) A.plastic_next[6 * tet + i] = pl[i];
       }
       // |V| D is stored; the reference multiplies D by the signed V, which is
       // positive for every accepted mesh.
@@ -1180,8 +1183,20 @@ hipError_t enqueue_mode(const StepArgs &a, int mode, size_t lds_cg, hipStream_t 
 
 }  // namespace
 
+namespace {
+
+// plastic := plastic_next once every tile has finished k_element.
+__global__ __launch_bounds__(256) void k_plastic_commit(StepArgs A) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < 6 * A.T) A.plastic[i] = A.plastic_next[i];
+}
+
+}  // namespace
+
 hipError_t enqueue_element(const StepArgs &a, size_t lds_elem, hipStream_t s) {
   hipLaunchKernelGGL(k_element, dim3(a.ntiles), dim3(kElemBlock), lds_elem, s, a);
+  if (a.plastic_on && a.T > 0)
+    hipLaunchKernelGGL(k_plastic_commit, dim3((unsigned)((6 * a.T + 255) / 256)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
