@@ -105,6 +105,30 @@ def test_per_step_parity_with_oracle(mode):
     assert max(errs) <= 1e-6
 
 
+def test_plastic_strain_shared_by_many_tiles_same_bits():
+    """Tiles of about 16 nodes put most tets in several tiles, each computing
+    the tet's plastic update; every copy must read the strain of the start of
+    the step, whatever order the tiles run in."""
+    from opentissue_amd import fem
+    c, t, fx, f, E, kw = _cube(9, seed=5, plastic=True)
+    P = fem.config_params(cap=3000)
+
+    def run():
+        s = fem.Solver(c, t, fx, target_tile_nodes=16)
+        s.init_materials(E, 0.3, 1000.0, **kw)
+        s.set_external_forces(f)
+        s.set_deterministic(True)
+        for _ in range(6):
+            s.simulate(0.005, True, P)
+        out = s.state() + (s.plastic(),)
+        s.close()
+        return out
+
+    a, b = run(), run()
+    assert np.abs(a[2]).max() > 0
+    assert all(np.array_equal(p, q) for p, q in zip(a, b))
+
+
 def test_learned_start_and_checkpoint_resume_same_bits(tmp_path):
     from opentissue_amd import fem
     c, t, fx, f, E, kw = _cube(7, seed=2, plastic=True)
