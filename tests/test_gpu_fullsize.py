@@ -1,0 +1,131 @@
+"""The workloads at their full sizes: the jittered 64^3 cube (1,310,720 tets,
+274,625 nodes), 512 uneven bodies (~10 M tets) and 2,048 uneven bodies
+(~40 M tets).  The oracle needs about 10 s for one step of the cube, so the
+cube's first step is checked against it directly; beyond that the checks are
+properties that hold at any size: a rigid motion of the whole problem moves
+the result with it, a body stepped inside a batch ends where the same body
+stepped alone ends, and every body of the batched sets converges.
+
+The rigid-motion and batch checks run CG to rel-res 1e-9, so the two runs
+compared cannot stop one iteration apart at the 1e-6 threshold (which alone
+moves a step by about 1e-6); the bar is the 1e-6 per-step displacement bar."""
+import numpy as np
+import pytest
+
+from conftest import rel_disp_error
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+
+
+def _rotation(seed):
+    q, _ = np.linalg.qr(np.random.default_rng(seed).normal(size=(3, 3)))
+    return q * np.sign(np.linalg.det(q))
+
+
+def _solver(w, coords=None, forces=None, nodes=None, tets=None):
+    from opentissue_amd import fem
+    if nodes is None:
+        s = fem.Solver(w.coords if coords is None else coords, w.tets, w.fixed,
+                       body_node_offsets=w.body_node_offsets if w.num_bodies > 1 else None,
+                       body_tet_offsets=w.body_tet_offsets if w.num_bodies > 1 else None)
+        s.init_materials(w.young, w.poisson, w.density)
+        s.set_external_forces(w.forces if forces is None else forces)
+        return s
+    s = fem.Solver(w.coords[nodes], w.tets[tets] - nodes.start, w.fixed[nodes])
+    s.init_materials(w.young[tets], w.poisson[tets], w.density[tets])
+    s.set_external_forces(w.forces[nodes])
+    return s
+
+
+def test_cube64_first_step_matches_oracle():
+    from opentissue_amd import fem, workloads
+    import oracle
+    w = workloads.cube64()
+    assert w.num_tets == 1310720 and w.num_nodes == 274625
+    s = _solver(w)
+    st = s.simulate(w.dt, True, fem.config_params(cap=5000))
+    x, _ = s.state()
+    s.close()
+    o = oracle.Oracle(w.coords, w.tets, w.fixed)
+    o.init_materials(w.young, w.poisson, w.density)
+    o.set_external_forces(w.forces)
+    ost = o.simulate(w.dt, True, oracle.config_params(cap=5000))
+    err = rel_disp_error(x, o.state()[0], w.coords)
+    print("cube64 step 1: iterations gpu", st.iterations, "oracle", ost.iterations, "error", err,
+          flush=True)
+    assert st.bodies_converged == 1 and ost.converged
+    # FP32 slot records round the operator, so near the threshold the two
+    # solves can stop a few iterations apart (301 vs 305 measured)
+    assert abs(st.iterations - ost.iterations) <= 0.05 * ost.iterations
+    assert err <= TOL
+
+
+def test_cube64_rigid_motion_moves_the_result():
+    """x'(Q X + t) = Q x(X) + t for Q a rotation: polar rotations (exactly
+    equivariant), three steps, different tilings of the two meshes."""
+    from opentissue_amd import fem, workloads
+    w = workloads.cube64()
+    Q, t = _rotation(11), np.array([0.3, -1.2, 0.7])
+    P = fem.config_params(tol=1e-9, cap=5000, rotation_mode=1)
+    a = _solver(w)
+    b = _solver(w, coords=w.coords @ Q.T + t, forces=w.forces @ Q.T)
+    errs = []
+    for _ in range(3):
+        sa = a.simulate(w.dt, True, P)
+        sb = b.simulate(w.dt, True, P)
+        assert sa.bodies_converged == 1 and sb.bodies_converged == 1
+        xa, va = a.state()
+        xb, vb = b.state()
+        errs.append(rel_disp_error(xb, xa @ Q.T + t, w.coords @ Q.T + t))
+    print("rigid motion errors", errs, flush=True)
+    assert np.linalg.norm(xa - w.coords, axis=1).max() > 1e-6  # the cube moved
+    assert max(errs) <= TOL
+    assert np.abs(vb - va @ Q.T).max() <= TOL * np.abs(va).max()
+    a.close()
+    b.close()
+
+
+def _batch_vs_alone(w, picks, steps, P):
+    s = _solver(w)
+    for _ in range(steps):
+        s.simulate(w.dt, True, P)
+    x, _ = s.state()
+    it, rel = s.body_stats()
+    unconv = s.unconverged_totals()
+    s.close()
+    assert np.isfinite(x).all()
+    assert not np.any(unconv), np.flatnonzero(unconv)[:8]
+    no, to = w.body_node_offsets, w.body_tet_offsets
+    for b in picks:
+        nodes, tets = slice(int(no[b]), int(no[b + 1])), slice(int(to[b]), int(to[b + 1]))
+        a = _solver(w, nodes=nodes, tets=tets)
+        for _ in range(steps):
+            a.simulate(w.dt, True, P)
+        xa, _ = a.state()
+        a.close()
+        err = rel_disp_error(x[nodes], xa, w.coords[nodes])
+        print("body", b, "tets", tets.stop - tets.start, "batch vs alone", err, flush=True)
+        assert err <= TOL
+    return it
+
+
+def test_bodies512_each_body_as_if_alone():
+    from opentissue_amd import fem, workloads
+    w = workloads.bodies()
+    assert w.num_bodies == 512
+    sizes = np.diff(w.body_tet_offsets)
+    picks = [0, int(np.argmin(sizes)), int(np.argmax(sizes)), 257]
+    it = _batch_vs_alone(w, picks, 2, fem.config_params(tol=1e-9, cap=5000))
+    assert (it > 0).all()
+
+
+def test_combined_every_body_converges_and_a_sample_runs_as_if_alone():
+    from opentissue_amd import fem, workloads
+    w = workloads.combined()
+    assert w.num_bodies == 2048 and w.num_tets > 30_000_000
+    sizes = np.diff(w.body_tet_offsets)
+    picks = [int(np.argmax(sizes)), 1024]
+    it = _batch_vs_alone(w, picks, 1, fem.config_params(tol=1e-9, cap=5000))
+    assert (it > 0).all()
