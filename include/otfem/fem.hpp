@@ -110,7 +110,10 @@ namespace detail {
 inline void check(otfem_status st, otfem_handle h, const char *what) {
   if (st == OTFEM_OK) return;
   std::string msg = std::string(what) + ": " + otfem_status_string(st);
-  if (h) msg += std::string(" (") + otfem_last_error(h) + ")";
+  if (h)
+    msg += std::string(" (") + otfem_last_error(h) + ")";
+  else if (*otfem_create_last_error())
+    msg += std::string(" (") + otfem_create_last_error() + ")";
   if (st == OTFEM_INVALID_ARG) throw std::invalid_argument(msg);
   throw std::runtime_error(msg);
 }

@@ -111,6 +111,9 @@ otfem_status otfem_device_count(int *count);
  * all solver state the reference keeps in node/tet traits
  * (fem_node_traits.h:37-52, fem_tetrahedron_traits.h:29-48). */
 otfem_status otfem_create(const otfem_mesh_desc *desc, otfem_handle *out);
+/* Why the calling thread's last otfem_create failed ("" after a success); the
+ * reference has no counterpart (its mesh building asserts or throws). */
+const char *otfem_create_last_error(void);
 otfem_status otfem_destroy(otfem_handle h);
 const char *otfem_last_error(otfem_handle h);
 

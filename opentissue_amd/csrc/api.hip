@@ -685,11 +685,25 @@ otfem_status otfem_device_count(int *count) {
   return OTFEM_OK;
 }
 
+// Why the calling thread's last otfem_create failed (no handle exists then).
+static thread_local std::string g_create_err;
+
+const char *otfem_create_last_error(void) { return g_create_err.c_str(); }
+
 otfem_status otfem_create(const otfem_mesh_desc *d, otfem_handle *out) {
-  if (!d || !out || !d->coord || !d->tets || d->num_nodes <= 0 || d->num_tets < 0)
+  g_create_err.clear();
+  auto bad = [](const char *why) {
+    try {
+      g_create_err = why;
+    } catch (...) {
+    }
     return OTFEM_INVALID_ARG;
+  };
+  if (!d || !out) return bad("null descriptor or output handle");
+  if (!d->coord || !d->tets) return bad("coordinates and tetrahedra are required");
+  if (d->num_nodes <= 0 || d->num_tets < 0) return bad("need at least one node and no negative tet count");
   *out = nullptr;
-  return no_throw(nullptr, [&]() -> otfem_status {
+  const otfem_status st = no_throw(nullptr, [&]() -> otfem_status {
     std::unique_ptr<otfem_solver> h(new otfem_solver());
     const int64_t N = d->num_nodes, T = d->num_tets;
     h->x_init.assign(d->coord, d->coord + 3 * N);
@@ -706,15 +720,19 @@ otfem_status otfem_create(const otfem_mesh_desc *d, otfem_handle *out) {
     for (auto &f : h->fixed_host) f = f ? 1 : 0;
     h->tets.assign(d->tets, d->tets + 4 * T);
     for (double c : h->x_init)
-      if (!std::isfinite(c)) return OTFEM_INVALID_ARG;
+      if (!std::isfinite(c)) return bad("non-finite coordinate");
     int B = d->num_bodies > 1 ? d->num_bodies : 1;
-    if (B > 1 && (!d->body_node_offsets || !d->body_tet_offsets)) return OTFEM_INVALID_ARG;
+    if (B > 1 && (!d->body_node_offsets || !d->body_tet_offsets))
+      return bad("num_bodies > 1 needs body_node_offsets and body_tet_offsets");
     std::string e = build_layout(N, T, h->x0_host.data(), h->tets.data(), B, d->body_node_offsets,
                                  d->body_tet_offsets, d->target_tile_nodes, 16384, h->L);
-    if (!e.empty()) return OTFEM_INVALID_ARG;
+    if (!e.empty()) return bad(e.c_str());
     *out = h.release();
     return OTFEM_OK;
   });
+  if (st != OTFEM_OK && g_create_err.empty())
+    bad(st == OTFEM_OUT_OF_MEMORY ? "out of host memory" : "mesh layout failed");
+  return st;
 }
 
 otfem_status otfem_destroy(otfem_handle h) {

@@ -124,6 +124,7 @@ SIGNATURES = {
     "otfem_create": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]),
     "otfem_destroy": (C.c_int, [_H]),
     "otfem_last_error": (C.c_char_p, [_H]),
+    "otfem_create_last_error": (C.c_char_p, []),
     "otfem_init": (C.c_int, [_H] + [C.c_double] * 6),
     "otfem_init_materials": (C.c_int, [_H, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double]),
     "otfem_set_external_forces": (C.c_int, [_H, _dp]),
@@ -303,8 +304,8 @@ class Solver:
         st = self.lib.otfem_create(C.byref(d), C.byref(h))  # copies what d points to
         del keep
         if st != OK:
-            raise OtfemError(st, "otfem_create failed: " + ("out of host memory" if st == OUT_OF_MEMORY
-                                                           else "invalid mesh description"))
+            raise OtfemError(st, "otfem_create failed: "
+                             + (self.lib.otfem_create_last_error() or b"").decode())
         self.h = h
 
     def close(self):

@@ -55,7 +55,7 @@ def test_create_validates_input(fem):
     c, t = mesh.generate_blocks(2, 2, 2, 0.1, 0.1, 0.1)
     bad = t.copy()
     bad[3, 2] = len(c)
-    with pytest.raises(fem.OtfemError):
+    with pytest.raises(fem.OtfemError, match="otfem_create failed: .+"):
         fem.Solver(c, bad)
     bad[3, 2] = -1
     with pytest.raises(fem.OtfemError):
@@ -71,6 +71,23 @@ def test_create_validates_input(fem):
     cross[0, 0] = no[1]  # body 0 tet pointing into body 1
     with pytest.raises(fem.OtfemError):
         fem.Solver(c2, cross, body_node_offsets=no, body_tet_offsets=to)
+
+
+def test_create_reports_why_it_failed(fem):
+    from opentissue_amd import mesh
+    lib = fem.load_library()
+    c, t = mesh.generate_blocks(30, 30, 30, 0.01, 0.01, 0.01)
+    # one tile of 29,791 nodes cannot fit a workgroup's LDS
+    with pytest.raises(fem.OtfemError, match="LDS budget"):
+        fem.Solver(c, t, target_tile_nodes=100000)
+    nan = c.copy()
+    nan[5, 1] = np.nan
+    with pytest.raises(fem.OtfemError, match="non-finite coordinate"):
+        fem.Solver(nan, t)
+    fem.Solver(c, t).close()
+    assert lib.otfem_create_last_error() == b""  # cleared by a success
+    assert lib.otfem_create(None, None) == fem.INVALID_ARG
+    assert b"null" in lib.otfem_create_last_error()
 
 
 @pytest.mark.parametrize("dims,tile", [((10, 40, 10), 0), ((6, 7, 5), 64), ((1, 1, 1), 0)])
