@@ -346,6 +346,38 @@ def main():
         "cpu_baseline": None,
     }
 
+    # A caller that keeps the node arrays in host memory (the fem.hpp drop-in)
+    # uploads x, v and f_ext and reads x, v back every step.  The same first
+    # steps from rest, device-resident and through host buffers; never `value`.
+    k2 = min(args.steps, 20)
+    s.set_timing(False)
+    s.set_state(w.coords, zeros)
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(k2):
+        s.step_async(w.dt, True, P)
+    s.synchronize()
+    dev_s = time.perf_counter() - t0
+    xh, vh = w.coords.copy(), zeros.copy()
+    t0 = time.perf_counter()
+    for _ in range(k2):
+        s.set_state(xh, vh)
+        s.set_external_forces(w.forces)
+        s.simulate(w.dt, True, P)
+        xh, vh = s.state()
+    host_s = time.perf_counter() - t0
+    result["host_buffers"] = {
+        "value": T * k2 / host_s,
+        "unit": "tet-steps/s",
+        "device_resident_same_steps": T * k2 / dev_s,
+        "steps": k2,
+        "note": "steps 1..%d from rest with x, v, f_ext copied from pageable host arrays and x, v "
+                "copied back every step (%.1f MB up, %.1f MB down), against the same steps "
+                "device-resident" % (k2, 9 * 8 * N / 1e6, 6 * 8 * N / 1e6),
+    }
+    log(f"host-buffer caller: {T * k2 / host_s:.4g} tet-steps/s vs {T * k2 / dev_s:.4g} "
+        f"device-resident over the first {k2} steps")
+
     if not args.no_learned:
         from opentissue_amd import fem as F
         log("learned warm start: recording 2,000 cantilever steps and training (not timed)")
