@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 EXE = os.path.join(ROOT, "tests", "cpp", "build", "dropin_c1")
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_cpp_dropin_matches_oracle_per_step(tmp_path, mode):
     import oracle
     from opentissue_amd import mesh
@@ -31,20 +31,31 @@ def test_cpp_dropin_matches_oracle_per_step(tmp_path, mode):
     states = data[:(steps + 1) * 6 * N].reshape(steps + 1, 2, N, 3)
     nk = int(data[(steps + 1) * 6 * N])
     fixed = (c[:, 2] == 0).astype(np.uint8)
+    fext = data[(steps + 1) * 6 * N + 1:].reshape(steps, N, 3)
     o = oracle.Oracle(c, t, fixed)
-    o.init(5e5, 0.33, 1000.0)
+    if mode == 2:
+        o.init(5e5, 0.33, 1000.0, 1e-4, 20.0, 0.05)
+    else:
+        o.init(5e5, 0.33, 1000.0)
     m = o.mass()
-    o.set_external_forces(mesh.gravity_forces(np.where(fixed == 1, 0.0, m), fixed))
+    f = mesh.gravity_forces(np.where(fixed == 1, 0.0, m), fixed) * (50.0 if mode == 2 else 1.0)
     P = oracle.reference_params() if mode == 0 else oracle.config_params()
     np.testing.assert_array_equal(states[0, 0], c)
     errs = []
     for k in range(steps):
+        # the caller resets m_f_external before every step
+        o.set_external_forces(f)
         o.set_state(states[k, 0], states[k, 1])
         o.simulate(0.01, True, P)
         xo, _ = o.state()
         errs.append(rel_disp_error(states[k + 1, 0], xo, c))
+        # m_f_external read back: the step's plastic forces added to the load
+        np.testing.assert_allclose(fext[k], o.external_forces(), rtol=1e-6,
+                                   atol=1e-9 * np.abs(f).max())
     print("per-step errors", errs)
     assert max(errs) < 1e-6
+    if mode == 2:
+        assert np.abs(fext[-1] - f).max() > 1e-3 * np.abs(f).max()  # plasticity acted
     # fem::export_K: an identity entry per fixed dof plus 9 per free-free block
     Kd = o.export_K_dense()
     fr = np.repeat(fixed == 0, 3)
