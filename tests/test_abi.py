@@ -199,3 +199,15 @@ def test_blocks_reject_bad_face_axis(fem):
         None, None, off.ctypes.data_as(C.POINTER(C.c_int64)), off.ctypes.data_as(C.POINTER(C.c_int64)), None)
     assert st == fem.INVALID_ARG
     assert b"face axis" in fem.load_library().otfem_blocks_last_error()
+
+
+def test_missing_library_fails_loudly(monkeypatch):
+    """Without the built HIP library nothing falls back: every call raises."""
+    from opentissue_amd import fem, mesh
+    monkeypatch.setattr(fem, "_lib", None)
+    monkeypatch.setattr(fem, "LIB_PATH", "/nonexistent/libotfem.so")
+    c, t = mesh.generate_blocks(1, 1, 1, 0.01, 0.01, 0.01)
+    with pytest.raises(fem.OtfemError, match="missing"):
+        fem.Solver(c, t)
+    with pytest.raises(fem.OtfemError, match="missing"):
+        fem.config_params()
