@@ -84,17 +84,23 @@ void parallel_for(int64_t n, int threads, F &&f) {
 
 // Picks an order for the slots of one tile, in windows of 64 (one wave
 // instruction) in which no two slots share an owned corner, so the per-corner
-// LDS atomics of a wave never hit the same address.  Each half window (32 lanes) also spreads
-// each corner's node over the 32 residues of its local index mod 32: a tile
-// node's three FP64 values start at 8-byte word 3 l, so nodes of one residue
-// hit the same LDS banks, and lanes of a half wave that read different nodes
-// of one residue wait for each other.  Half windows are filled in passes,
-// each taking compatible slots in order: at most 1, 2, 3 and then 4 distinct
-// nodes per corner and residue (looking at most kLook slots past the first
-// remaining one), then any slot without a shared owned corner, then any slot,
-// so later windows stay aligned.
+// LDS atomics of a wave never hit the same address.  Each quarter window (16
+// lanes) also spreads each corner's node over the 16 residues of its local
+// index mod 16: a tile node's three FP64 values start at 8-byte word 3 l, an
+// 8-byte access of 16 lanes is served in one pass when their words fall in
+// distinct bank pairs (word mod 16), so lanes of a quarter wave that read
+// different nodes of one residue wait for each other.  (Measured on the 64^3
+// cube: 32.6-33.1 us per CG iteration with quarter windows mod 16, 33.1 us
+// with half windows mod 32 -- equal within the spread between runs -- and
+// 36.0 us without spreading.)  Quarter windows are filled in
+// passes, each taking compatible slots in order: at most 1, 2, 3 and then 4
+// distinct nodes per corner and residue (looking at most kLook slots past the
+// first remaining one), then any slot without a shared owned corner, then any
+// slot, so later windows stay aligned.
 constexpr int kLook = 512;
 constexpr int kCaps = 4;
+constexpr int kGroup = 16;  // lanes served by the LDS in one pass
+constexpr int kRes = 16;    // bank residues of a node's local index
 
 void pick_slots(const uint16_t *sc, int32_t n, int32_t n_owned, std::vector<int32_t> &order) {
   order.clear();
@@ -103,21 +109,21 @@ void pick_slots(const uint16_t *sc, int32_t n, int32_t n_owned, std::vector<int3
   std::vector<int32_t> mark(n_owned, -1);
   int32_t first = 0;
   for (int32_t window = 0; (int32_t)order.size() < n; ++window) {
-    for (int half = 0; half < 2 && (int32_t)order.size() < n; ++half) {
-      // per corner and residue: distinct nodes taken in this half (first kCaps kept)
-      int count[4][32] = {};
-      int32_t seen[4][32][kCaps];
+    for (int part = 0; part < 64 / kGroup && (int32_t)order.size() < n; ++part) {
+      // per corner and residue: distinct nodes taken in this part (first kCaps kept)
+      int count[4][kRes] = {};
+      int32_t seen[4][kRes][kCaps];
       auto fresh = [&](int c, int32_t li) {  // li not yet taken at corner c
-        const int r = li & 31;
+        const int r = li & (kRes - 1);
         for (int q = 0; q < std::min(count[c][r], kCaps); ++q)
           if (seen[c][r][q] == li) return false;
         return true;
       };
       int got = 0;
-      for (int pass = 0; pass < kCaps + 2 && got < 32; ++pass) {
+      for (int pass = 0; pass < kCaps + 2 && got < kGroup; ++pass) {
         const int cap = pass + 1;  // passes 0..kCaps-1: residue cap 1..kCaps
         int looked = 0;
-        for (int32_t k = first; k < n && got < 32; ++k) {
+        for (int32_t k = first; k < n && got < kGroup; ++k) {
           if (taken[k]) continue;
           if (pass < kCaps && ++looked > kLook) break;
           bool ok = true;
@@ -129,13 +135,13 @@ void pick_slots(const uint16_t *sc, int32_t n, int32_t n_owned, std::vector<int3
           if (pass < kCaps)
             for (int c = 0; c < 4 && ok; ++c) {
               const int32_t li = sc[4 * k + c];
-              if (fresh(c, li) && count[c][li & 31] >= cap) ok = false;
+              if (fresh(c, li) && count[c][li & (kRes - 1)] >= cap) ok = false;
             }
           if (!ok) continue;
           for (int c = 0; c < 4; ++c) {
             const int32_t li = sc[4 * k + c];
             if (li < n_owned) mark[li] = window;
-            const int r = li & 31;
+            const int r = li & (kRes - 1);
             if (fresh(c, li)) {
               if (count[c][r] < kCaps) seen[c][r][count[c][r]] = li;
               ++count[c][r];
@@ -153,9 +159,7 @@ void pick_slots(const uint16_t *sc, int32_t n, int32_t n_owned, std::vector<int3
 
 // The slot order of one tile: two rounds of pick_slots, the second over the
 // first's sequence, whose windows already hold compatible slots side by side
-// (the second round spreads the residues further: mean largest count of
-// distinct nodes per corner and residue in a half window 2.6 after one round,
-// 2.2 after two, on the 64^3 cube's tiles).
+// (the second round spreads the residues further).
 void order_slots(const TileInfo &ti, uint16_t *slot_conn, int32_t *slot_tet) {
   const int32_t n = ti.n_slots;
   const uint16_t *sc = slot_conn + 4 * (int64_t)ti.slot_begin;
