@@ -57,13 +57,15 @@ def config_params(alpha=0.1, beta=0.001, tol=1e-6, cap=200, tiny=0.0, rotation_m
     return OracleParams(alpha, beta, 0, cap, 1, tol, tiny, rotation_mode)
 
 
-def load_library():
+def load_library(path=None):
+    """The oracle library (built on first use), or the library at `path`
+    (a variant of the oracle built elsewhere, e.g. with a planted mistake)."""
     global _lib
-    if _lib is not None:
+    if path is None and _lib is not None:
         return _lib
-    if not os.path.exists(_LIB_PATH):
+    if path is None and not os.path.exists(_LIB_PATH):
         build()
-    lib = C.CDLL(_LIB_PATH)
+    lib = C.CDLL(_LIB_PATH if path is None else path)
     P = C.c_void_p
     dp = C.POINTER(C.c_double)
     ip = C.POINTER(C.c_int)
@@ -101,7 +103,8 @@ def load_library():
     lib.orc_polar_decomposition.restype = C.c_int
     lib.orc_polar_decomposition.argtypes = [dp, dp, dp]
     lib.orc_default_params.argtypes = [C.POINTER(OracleParams)]
-    _lib = lib
+    if path is None:
+        _lib = lib
     return lib
 
 
@@ -119,8 +122,8 @@ def _c64(a, shape=None):
 class Oracle:
     """One mesh in the oracle.  Mirrors fem::init / fem::simulate."""
 
-    def __init__(self, coords, tets, fixed=None, velocity=None, model_coords=None):
-        self.lib = load_library()
+    def __init__(self, coords, tets, fixed=None, velocity=None, model_coords=None, lib=None):
+        self.lib = load_library() if lib is None else lib
         coords = _c64(coords)
         self.N = coords.shape[0]
         tets = np.ascontiguousarray(tets, dtype=np.int32)
