@@ -66,6 +66,12 @@ typedef struct {
   int rotation_mode;     /* OTFEM_ROT_* (ref: GRAM_SCHMIDT) */
   int warm_start;        /* OTFEM_WARM_* (ref: PREVIOUS_VELOCITY) */
   int store_rotations;   /* 1: keep per-tet Re for otfem_get_rotations */
+  int subspace;          /* 0 (ref): CG starts from the warm start above.  1..10:
+                            subspace start -- CG starts from v^n + D c, where D holds
+                            v^n, the differences of the last `subspace` velocities
+                            and, with OTFEM_WARM_LEARNED, the network's start minus
+                            v^n, and c minimises the A-norm error of the start over
+                            their span (per body); no reference counterpart */
 } otfem_params;
 
 typedef struct {
@@ -152,6 +158,10 @@ otfem_status otfem_synchronize(otfem_handle h, otfem_stats *stats);
 otfem_status otfem_set_timing(otfem_handle h, int enable);
 otfem_status otfem_get_timing(otfem_handle h, double *element_ms, double *cg_ms,
                               double *network_ms, long long *steps);
+/* The same accumulation with the subspace start's kernels (Gram partials,
+ * start and its residual) apart: ms[0] network, ms[1] element kernel,
+ * ms[2] subspace start, ms[3] CG.  No reference counterpart. */
+otfem_status otfem_get_timing_detail(otfem_handle h, double *ms, long long *steps);
 /* CG iterations per body summed over all steps since the last reset
  * (reset != 0 clears after reading; totals may be NULL).  No reference
  * counterpart; used to count algorithmic bytes of asynchronous steps. */
@@ -207,6 +217,10 @@ otfem_status otfem_autotune_layout(const otfem_mesh_desc *desc, const double *yo
 const char *otfem_autotune_layout_last_error(void);
 /* Per-body CG iterations and final relative residual of the last step. */
 otfem_status otfem_body_stats(otfem_handle h, int *iterations, double *rel_residual);
+/* Coefficients c of the last subspace start, [B][12] (body-major; c[0] is
+ * v^n's, then the differences from the newest, then the network's; unused
+ * entries 0).  No reference counterpart (diagnostics of this start). */
+otfem_status otfem_subspace_coefficients(otfem_handle h, double *coef);
 
 /* fem::update_original_coord (fem_update_original_coord.h:28-33): x0 := x.
  * As in the reference, otfem_init must be called again afterwards. */

@@ -86,7 +86,7 @@ struct otfem_solver {
   std::vector<BodyState> last_body;
   bool pending = false;  // asynchronous steps enqueued since the last sync
   double last_elem_ms = 0.0, last_cg_ms = 0.0;
-  double acc_elem_ms = 0.0, acc_cg_ms = 0.0, acc_ws_ms = 0.0;
+  double acc_elem_ms = 0.0, acc_cg_ms = 0.0, acc_ws_ms = 0.0, acc_sub_ms = 0.0;
   long long timed_steps = 0;
   bool timing = false;
   // per-step event triples of asynchronous timed steps, resolved at sync
@@ -102,6 +102,16 @@ struct otfem_solver {
   float4 *hist1 = nullptr, *hist2 = nullptr, *kd = nullptr;
   int32_t *free_rank = nullptr;
   int *ws_bad = nullptr;
+  // subspace start: ring of past velocities, its state, Gram partials
+  float4 *vring = nullptr;
+  double *vbuf = nullptr;
+  int ring_slots = 0;
+  int32_t *sub_state = nullptr;
+  double *sub_part = nullptr;
+  int32_t *body_tiles = nullptr;  // [B+1]
+  double *sub_coef = nullptr;     // [B][kSubMax]
+  size_t lds_sub = 0;
+  hipGraphExec_t g_sub = nullptr;
 };
 
 namespace {
@@ -162,6 +172,8 @@ constexpr int kCgTail = 8;    // iterations per graph near the expected stop
 void drop_graphs(otfem_solver *h) {
   if (h->g_elem) (void)hipGraphExecDestroy(h->g_elem);
   if (h->g_fin) (void)hipGraphExecDestroy(h->g_fin);
+  if (h->g_sub) (void)hipGraphExecDestroy(h->g_sub);
+  h->g_sub = nullptr;
   for (auto &kv : h->g_chunks)
     if (kv.second) (void)hipGraphExecDestroy(kv.second);
   h->g_elem = h->g_fin = nullptr;
@@ -180,6 +192,9 @@ void free_device(otfem_solver *h) {
   dfree(h->vw); dfree(h->hist1); dfree(h->hist2); dfree(h->kd); dfree(h->free_rank);
   dfree(h->ws_bad); dfree(h->act);
   dfree(h->ord_begin); dfree(h->ord_ent); dfree(h->ordered_terms);
+  dfree(h->vring); dfree(h->vbuf); dfree(h->sub_state); dfree(h->sub_part);
+  dfree(h->body_tiles); dfree(h->sub_coef);
+  h->ring_slots = 0;
   drop_graphs(h);
   if (h->done_host) (void)hipHostFree(h->done_host);
   h->done_host = nullptr;
@@ -236,9 +251,19 @@ otfem_status ws_upload_free(otfem_solver *h) {
   return OTFEM_OK;
 }
 
-// Velocity history of the learned start: zero = at rest before this state.
+// Velocity history of the subspace start: none before this state.
+otfem_status sub_reset_history(otfem_solver *h) {
+  if (!h->sub_state) return OTFEM_OK;
+  HIPCHK(h, hipMemsetAsync(h->sub_state, 0, 3 * sizeof(int32_t), h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return OTFEM_OK;
+}
+
+// Velocity histories of the learned and subspace starts: zero / none = at
+// rest before this state.
 otfem_status ws_reset_history(otfem_solver *h) {
-  if (!h->hist1) return OTFEM_OK;
+  otfem_status st = sub_reset_history(h);
+  if (st != OTFEM_OK || !h->hist1) return st;
   HIPCHK(h, hipMemsetAsync(h->hist1, 0, h->L.N * sizeof(float4), h->stream));
   HIPCHK(h, hipMemsetAsync(h->hist2, 0, h->L.N * sizeof(float4), h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -573,21 +598,74 @@ StepArgs make_args(otfem_solver *h, double dt, int warp, const otfem_params &P) 
   a.ord_begin = h->ord_begin;
   a.ord_ent = h->ord_ent;
   a.ordered_terms = h->ordered_terms;
+  a.sub_hist = P.subspace > 0 && h->vring ? P.subspace : 0;
+  a.vring = h->vring;
+  a.vbuf = h->vbuf;
+  a.sub_state = h->sub_state;
+  a.sub_ring = h->ring_slots;
+  a.sub_part = h->sub_part;
+  a.body_tiles = h->body_tiles;
+  a.sub_coef = h->sub_coef;
+  a.sub_net = 0;
+  a.vnet = nullptr;
   return a;
 }
 
+// Buffers of the subspace start for `hist` velocity differences (a ring of
+// hist slots and the last two velocities; a larger request reallocates the
+// ring and starts the history anew).
+otfem_status sub_ensure(otfem_solver *h, int hist, int directions) {
+  const int64_t N = h->L.N, nt = (int64_t)h->L.tiles.size(), B = h->L.B;
+  if (!h->sub_part) HIPCHK(h, dalloc(&h->sub_part, kSubPart * std::max<int64_t>(nt, 1)));
+  if (!h->sub_coef) {
+    HIPCHK(h, dalloc(&h->sub_coef, kSubMax * B));
+    HIPCHK(h, hipMemsetAsync(h->sub_coef, 0, kSubMax * B * sizeof(double), h->stream));
+  }
+  if (!h->body_tiles) {
+    HIPCHK(h, dalloc(&h->body_tiles, B + 1));
+    HIPCHK(h, hipMemcpyAsync(h->body_tiles, h->L.body_tile_begin.data(), (B + 1) * sizeof(int32_t),
+                             hipMemcpyHostToDevice, h->stream));
+  }
+  if (!h->sub_state) {
+    HIPCHK(h, dalloc(&h->sub_state, 3));
+    HIPCHK(h, hipMemsetAsync(h->sub_state, 0, 3 * sizeof(int32_t), h->stream));
+  }
+  if (!h->vbuf) HIPCHK(h, dalloc(&h->vbuf, 6 * N));
+  if (h->ring_slots < hist) {
+    // steps enqueued earlier may still read the old ring
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    drop_graphs(h);
+    dfree(h->vring);
+    h->ring_slots = 0;
+    HIPCHK(h, dalloc(&h->vring, (size_t)hist * N));
+    h->ring_slots = hist;
+    HIPCHK(h, hipMemsetAsync(h->sub_state, 0, 3 * sizeof(int32_t), h->stream));
+  }
+  const size_t lds = sub_gram_lds_bytes(h->L.max_tile_nodes, directions);
+  hipDeviceProp_t prop;
+  HIPCHK(h, hipGetDeviceProperties(&prop, h->device));
+  if (lds > (size_t)prop.sharedMemPerBlock)
+    return fail(h, OTFEM_INVALID_ARG,
+                "tile too large for the subspace start's LDS; use fewer directions or lower target_tile_nodes");
+  h->lds_sub = std::max(h->lds_sub, lds);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return OTFEM_OK;
+}
+
 otfem_status resolve_timing(otfem_solver *h) {
-  // four events per timed step: start, after the network, after the
-  // element kernel, after the CG kernels
-  for (size_t k = 0; 4 * k + 3 < h->pool_used; ++k) {
-    float e0 = 0.f, e1 = 0.f, e2 = 0.f;
-    HIPCHK(h, hipEventElapsedTime(&e0, h->pool[4 * k], h->pool[4 * k + 1]));
-    HIPCHK(h, hipEventElapsedTime(&e1, h->pool[4 * k + 1], h->pool[4 * k + 2]));
-    HIPCHK(h, hipEventElapsedTime(&e2, h->pool[4 * k + 2], h->pool[4 * k + 3]));
+  // five events per timed step: start, after the network, after the
+  // element kernel, after the subspace start, after the CG kernels
+  for (size_t k = 0; 5 * k + 4 < h->pool_used; ++k) {
+    float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&e0, h->pool[5 * k], h->pool[5 * k + 1]));
+    HIPCHK(h, hipEventElapsedTime(&e1, h->pool[5 * k + 1], h->pool[5 * k + 2]));
+    HIPCHK(h, hipEventElapsedTime(&e3, h->pool[5 * k + 2], h->pool[5 * k + 3]));
+    HIPCHK(h, hipEventElapsedTime(&e2, h->pool[5 * k + 3], h->pool[5 * k + 4]));
     h->acc_ws_ms += e0;
-    h->last_elem_ms = e1;
+    h->last_elem_ms = e1 + e3;
     h->last_cg_ms = e2;
     h->acc_elem_ms += e1;
+    h->acc_sub_ms += e3;
     h->acc_cg_ms += e2;
     h->timed_steps++;
   }
@@ -905,6 +983,13 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
     return fail(h, OTFEM_INVALID_ARG, "bad solver parameters");
   if (P.warm_start != OTFEM_WARM_PREVIOUS_VELOCITY && P.warm_start != OTFEM_WARM_LEARNED)
     return fail(h, OTFEM_INVALID_ARG, "unknown warm start");
+  if (P.subspace < 0 || P.subspace > kSubMax - 2)
+    return fail(h, OTFEM_INVALID_ARG, "subspace must be 0..10");
+  if (P.subspace > 0) {
+    otfem_status ss =
+        sub_ensure(h, P.subspace, 1 + P.subspace + (P.warm_start == OTFEM_WARM_LEARNED ? 1 : 0));
+    if (ss != OTFEM_OK) return ss;
+  }
   const bool ws_on = h->model && !h->ws_suspended;
   const bool learned = P.warm_start == OTFEM_WARM_LEARNED;
   if (learned && !ws_on) return fail(h, OTFEM_STATE, "no warm-start model attached");
@@ -938,18 +1023,25 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
   if (h->lc.spmv_mode == SPMV_F64_STORE64 && !h->bt8)
     HIPCHK(h, dalloc(&h->bt8, 3 * (size_t)h->L.slot_tet.size()));
   StepArgs a = make_args(h, dt, use_stiffness_warping, P);
-  if (learned) a.vw = h->vw;
+  if (learned && a.sub_hist > 0) {
+    // the network's start is one direction of the subspace start; the
+    // element kernel forms the plain residual
+    a.sub_net = 1;
+    a.vnet = h->vw;
+  } else if (learned) {
+    a.vw = h->vw;
+  }
   hipStream_t s = h->stream;
   const bool timed = (stats != nullptr) || h->timing;
-  hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t e[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   if (timed) {
-    while (h->pool.size() < h->pool_used + 4) {
+    while (h->pool.size() < h->pool_used + 5) {
       hipEvent_t ev;
       HIPCHK(h, hipEventCreate(&ev));
       h->pool.push_back(ev);
     }
-    for (int k = 0; k < 4; ++k) e[k] = h->pool[h->pool_used + k];
-    h->pool_used += 4;
+    for (int k = 0; k < 5; ++k) e[k] = h->pool[h->pool_used + k];
+    h->pool_used += 5;
     HIPCHK(h, hipEventRecord(e[0], s));
   }
   if (ws_on) HIPCHK(h, enqueue_ws_predict(w, s));
@@ -982,6 +1074,9 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
       if (cs == OTFEM_OK)
         cs = capture([&](hipStream_t cs_) { return enqueue_finish(a, h->lc, cs_); }, &h->g_fin,
                      "enqueue_finish");
+      if (cs == OTFEM_OK && a.sub_hist > 0)
+        cs = capture([&](hipStream_t cs_) { return enqueue_subspace(a, h->lds_sub, h->lds_cg, cs_); },
+                     &h->g_sub, "enqueue_subspace");
       if (cs != OTFEM_OK) {
         drop_graphs(h);
         return cs;
@@ -989,10 +1084,14 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
       h->graph_key = key;
     }
     HIPCHK(h, hipGraphLaunch(h->g_elem, s));
+    if (timed) HIPCHK(h, hipEventRecord(e[2], s));
+    if (h->g_sub) HIPCHK(h, hipGraphLaunch(h->g_sub, s));
   } else {
     HIPCHK(h, enqueue_element(a, h->lds_elem, s));
+    if (timed) HIPCHK(h, hipEventRecord(e[2], s));
+    HIPCHK(h, enqueue_subspace(a, h->lds_sub, h->lds_cg, s));
   }
-  if (timed) HIPCHK(h, hipEventRecord(e[2], s));
+  if (timed) HIPCHK(h, hipEventRecord(e[3], s));
   // CG in chunks: kCgChunk iterations, and kCgTail from 16 iterations before
   // the iteration at which the previous step was seen to stop (so the chunks
   // that run after the stop are short).  The host keeps one chunk queued
@@ -1054,7 +1153,7 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
     HIPCHK(h, enqueue_ws_target(w, s));
     model_commit(h->model, h->n_free);
   }
-  if (timed) HIPCHK(h, hipEventRecord(e[3], s));
+  if (timed) HIPCHK(h, hipEventRecord(e[4], s));
   h->rotations_valid = (P.store_rotations != 0) && use_stiffness_warping;
   h->pending = true;
   if (!stats) return OTFEM_OK;
@@ -1074,7 +1173,7 @@ otfem_status otfem_set_timing(otfem_handle h, int enable) {
     h->pool_used = 0;
   }
   h->timing = enable != 0;
-  h->acc_elem_ms = h->acc_cg_ms = h->acc_ws_ms = 0.0;
+  h->acc_elem_ms = h->acc_cg_ms = h->acc_ws_ms = h->acc_sub_ms = 0.0;
   h->timed_steps = 0;
   return OTFEM_OK;
 }
@@ -1087,10 +1186,21 @@ otfem_status otfem_get_timing(otfem_handle h, double *element_ms, double *cg_ms,
     otfem_status ts = resolve_timing(h);
     if (ts != OTFEM_OK) return ts;
   }
-  if (element_ms) *element_ms = h->acc_elem_ms;
+  if (element_ms) *element_ms = h->acc_elem_ms + h->acc_sub_ms;
   if (network_ms) *network_ms = h->acc_ws_ms;
   if (cg_ms) *cg_ms = h->acc_cg_ms;
   if (steps) *steps = h->timed_steps;
+  return OTFEM_OK;
+}
+
+otfem_status otfem_get_timing_detail(otfem_handle h, double *ms, long long *steps) {
+  if (!h || !ms) return OTFEM_INVALID_ARG;
+  otfem_status st = otfem_get_timing(h, nullptr, nullptr, nullptr, steps);
+  if (st != OTFEM_OK) return st;
+  ms[0] = h->acc_ws_ms;
+  ms[1] = h->acc_elem_ms;
+  ms[2] = h->acc_sub_ms;
+  ms[3] = h->acc_cg_ms;
   return OTFEM_OK;
 }
 
@@ -1219,6 +1329,7 @@ static otfem_status tune_blocks(otfem_handle h, double dt, int use_stiffness_war
   else
     otfem_default_params(&P, 0);
   P.warm_start = OTFEM_WARM_PREVIOUS_VELOCITY;
+  P.subspace = 0;
   params = &P;
   h->ws_suspended = true;
   otfem_status ret = OTFEM_OK;
@@ -1355,6 +1466,14 @@ otfem_status otfem_body_stats(otfem_handle h, int *iterations, double *rel_resid
     if (iterations) iterations[b] = s.iterations;
     if (rel_residual) rel_residual[b] = s.bb > 0 ? std::sqrt(s.rr / s.bb) : 0.0;
   }
+  return OTFEM_OK;
+}
+
+otfem_status otfem_subspace_coefficients(otfem_handle h, double *coef) {
+  if (!h || !coef) return OTFEM_INVALID_ARG;
+  if (!h->sub_coef) return fail(h, OTFEM_STATE, "no step with the subspace start yet");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(coef, h->sub_coef, kSubMax * h->L.B * sizeof(double), hipMemcpyDeviceToHost));
   return OTFEM_OK;
 }
 

@@ -94,7 +94,29 @@ struct StepArgs {
   const int32_t *ord_begin;  // [N+1] internal node -> first entry
   const int32_t *ord_ent;    // [4T] entries 4 * slot + corner, ascending per node
   double *ordered_terms;     // [S][36] element terms (3 x [4][3]); CG uses [S][12]
+  // Subspace (Galerkin) CG start: v_start = v^n + sum_j c_j d_j with the
+  // directions d_0 = v^n, d_j = v^(n-j+1) - v^(n-j) (j = 1..nh) and, with a
+  // network attached, d = v_net - v^n; c minimises the A-norm error of the
+  // start over their span, per body.  Off when sub_hist == 0.
+  float4 *vring;             // [sub_ring][N] velocity differences v^j - v^(j-1) (xyz, FP32)
+  double *vbuf;              // [2][3N] v^(n-1) and v^n (halves alternate by step)
+  int32_t *sub_state;        // [3] on the device: ring slot of this step's difference,
+                             //     earlier steps since the reset, half of vbuf with v^(n-1)
+  int32_t sub_ring;          // ring slots (>= sub_hist)
+  int32_t sub_hist;          // past velocities used (nh <= sub_hist)
+  int32_t sub_net;           // the network's start is one more direction (vnet)
+  const double *vnet;        // [3N] the network's CG start (with sub_net)
+  double *sub_part;          // [ntiles][kSubPart] per-tile Gram partials
+  const int32_t *body_tiles; // [B+1] body b owns tiles [body_tiles[b], body_tiles[b+1])
+  double *sub_coef;          // [B][kSubMax] the step's coefficients c per body
 };
+
+// Subspace start: at most kSubMax directions; a tile's partial record holds
+// the upper triangle of D^T A D (kSubPairs, entry (i, j), i <= j, at
+// j (j + 1) / 2 + i) followed by D^T r_plain (kSubMax).
+constexpr int kSubMax = 12;
+constexpr int kSubPairs = kSubMax * (kSubMax + 1) / 2;
+constexpr int kSubPart = kSubPairs + kSubMax;
 
 struct LaunchConfig {
   int cg_block = 256;   // 256, 512 or 1024 threads per tile in the CG kernels
@@ -111,6 +133,13 @@ hipError_t enqueue_element(const StepArgs &a, size_t lds_elem, hipStream_t s);
 hipError_t enqueue_cg(const StepArgs &a, const LaunchConfig &lc, size_t lds_cg, hipStream_t s,
                       int it0, int it1, int grid);
 hipError_t enqueue_finish(const StepArgs &a, const LaunchConfig &lc, hipStream_t s);
+// Subspace start (after enqueue_element, before the CG): per-tile Gram
+// partials of the directions, then per body the A-optimal coefficients, the
+// start v^n + D c, its residual r_plain - A D c (one SpMV) and the tile
+// partials of r.r; then the ring advances.  lds_gram / lds_start from
+// sub_gram_lds_bytes / cg_lds_bytes.
+hipError_t enqueue_subspace(const StepArgs &a, size_t lds_gram, size_t lds_start, hipStream_t s);
+size_t sub_gram_lds_bytes(int max_nodes, int directions);
 // Surface coupling: out[3i..] = sum_k w[i].k x[3 ids[i].k ..] for bound
 // vertices (ids[i].x >= 0, internal node ids), rest[3i..] for the others.
 hipError_t enqueue_surface_update(int64_t n, const int4 *ids, const double4 *w, const double *rest,

@@ -1054,6 +1054,412 @@ void k_cg_iter(StepArgs A, int it) {
   if (threadIdx.x == 0) reinterpret_cast<double4 *>(A.part4 + cur * 4 * (int64_t)nt)[tile] = tot;
 }
 
+// ---------------------------------------------------------------------------
+// Subspace (Galerkin) CG start.  The reference starts CG from v^n
+// (fem_conjugate_gradients.h:48-67); here the start is v^n + D c with the
+// directions D = [v^n, v^n - v^n-1, ..., v^n-nh+1 - v^n-nh (, v_net - v^n)]
+// and c = (D^T A D)^-1 D^T (b - A v^n), the start of smallest A-norm error
+// in v^n + span(D) (v^n itself is c = 0).  CG then runs to the same stopping
+// rule.  Three kernels:
+//   k_sub_gram   per tile, the partials of D^T A D and D^T r_plain: the
+//                element part is the bilinear form p^T K'_e q =
+//                eps(p)^T D_e eps(q), written as E~^T E~ with E~ = L_e eps
+//                (L_e^T L_e = cK D_e), so a wave forms 4 slots x 16
+//                directions of E~ per round and adds E~^T E~ with FP64 MFMA
+//                (v_mfma_f64_16x16x4f64, 6 per round); the mass part
+//                likewise from rows sqrt(cM m) d.
+//   k_sub_solve  per body, the Gram sums in tile order and the coefficients.
+//   k_sub_start  per tile, w = D c, K' w (one SpMV), v = v^n + w and
+//                r = r_plain - A w, with the tile partials of r.r.
+// Precision: the velocity differences are kept in FP32 (their rounding is
+// relative to the difference, not to v), v^n in FP64, and every Gram product
+// and sum is FP64, so nearly dependent directions still give usable
+// coefficients; the start and its residual are FP64 and exactly consistent
+// (every tile forms w of a node from the same values).
+
+namespace {
+
+constexpr int kSubGramBlock = 256;
+constexpr int kSubStartBlock = 256;
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int sub_pair(int i, int j) { return j * (j + 1) / 2 + i; }
+
+// The step's history: ring slot of this step's newest difference, earlier
+// steps since the last reset, the half of vbuf with v^(n-1), the number of
+// differences used and of directions.
+struct SubState {
+  int head, prev, par, nh, k;
+};
+
+__device__ __forceinline__ SubState sub_state(const StepArgs &A) {
+  SubState s;
+  s.head = A.sub_state[0];
+  s.prev = A.sub_state[1];
+  s.par = A.sub_state[2];
+  s.nh = min(min(s.prev, A.sub_hist), A.sub_ring);
+  s.k = 1 + s.nh + (A.sub_net ? 1 : 0);
+  return s;
+}
+
+// Direction j >= 1 at internal node g as FP32: d_1 = v^n - v^(n-1) (given),
+// the older differences from the ring, the network's start - v^n.
+__device__ __forceinline__ void sub_dir(const StepArgs &A, int64_t g, const SubState &q, int j,
+                                        const float d1[3], const double vn[3], float d[3]) {
+  if (j == 1 && q.nh >= 1) {
+    d[0] = d1[0];
+    d[1] = d1[1];
+    d[2] = d1[2];
+  } else if (j <= q.nh) {
+    int slot = q.head - j + 1;
+    if (slot < 0) slot += A.sub_ring;
+    const float4 h = A.vring[(int64_t)slot * A.N + g];
+    d[0] = h.x;
+    d[1] = h.y;
+    d[2] = h.z;
+  } else {
+#pragma unroll
+    for (int x = 0; x < 3; ++x) d[x] = (float)(A.vnet[3 * g + x] - vn[x]);
+  }
+}
+
+// Per-tile partials of D^T A D (A = cM M + cK K') and D^T r_plain.  Owned
+// nodes also store v^n (FP64) and v^n - v^(n-1) (FP32, ring slot `head`); no
+// kernel of this step reads either before the kernel boundary.
+template <typename ST>
+__global__ __launch_bounds__(kSubGramBlock) void k_sub_gram(StepArgs A) {
+  extern __shared__ float sdir[];  // [nn][k][3]
+  constexpr int W = kSubGramBlock / 64;
+  __shared__ double cred[W][256];
+  __shared__ double gred[W][16];
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const TileInfo ti = A.tiles[tile];
+  const SubState q = sub_state(A);
+  const int k = q.k;
+  const int nn = ti.n_owned + ti.n_halo, no = ti.n_owned;
+  const double *vprev = A.vbuf + (int64_t)q.par * 3 * A.N;
+  double *vcur = A.vbuf + (int64_t)(1 - q.par) * 3 * A.N;
+  for (int l = threadIdx.x; l < nn; l += kSubGramBlock) {
+    const int64_t g = l < no ? ti.node_begin + l : A.halo[ti.halo_begin + l - no];
+    const double vn[3] = {A.v[3 * g], A.v[3 * g + 1], A.v[3 * g + 2]};
+    float d1[3] = {0.f, 0.f, 0.f};
+    if (q.prev >= 1)
+#pragma unroll
+      for (int x = 0; x < 3; ++x) d1[x] = (float)(vn[x] - vprev[3 * g + x]);
+    if (l < no) {
+#pragma unroll
+      for (int x = 0; x < 3; ++x) vcur[3 * g + x] = vn[x];
+      if (q.prev >= 1) A.vring[(int64_t)q.head * A.N + g] = make_float4(d1[0], d1[1], d1[2], 0.f);
+    }
+    const bool fx = A.fixed[g] != 0;
+    float *o = sdir + (int64_t)l * k * 3;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) o[x] = fx ? 0.f : (float)vn[x];
+    for (int j = 1; j < k; ++j) {
+      float d[3];
+      sub_dir(A, g, q, j, d1, vn, d);
+#pragma unroll
+      for (int x = 0; x < 3; ++x) o[3 * j + x] = fx ? 0.f : d[x];
+    }
+  }
+  __syncthreads();
+
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int qs = lane >> 4, dir = lane & 15;  // lane = (slot of the round, direction)
+  v4d C = {0.0, 0.0, 0.0, 0.0};
+  double Dt[3] = {0.0, 0.0, 0.0};
+  if (std::is_same<ST, Folded>::value) {
+    const double4 td = A.tile_d[tile];
+    Dt[0] = td.x;
+    Dt[1] = td.y;
+    Dt[2] = td.z;
+  }
+  const int64_t S = A.S;
+  const double cK = A.cK, cM = A.cM;
+  const int send = ti.slot_begin + ti.n_slots;
+  // element part: E~ rows of 4 slots per wave and round (wave-uniform loop)
+  for (int base = ti.slot_begin + 4 * wave; base < send; base += 4 * W) {
+    const int s = base + qs;
+    double e[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (s < send && dir < k && A.slot_tet[s] < 0) {  // each tet once: its primary copy
+      SlotRaw<ST> raw;
+      load_raw<ST>(A, S, s, raw);
+      double bt[3][3], D[3];
+      unpack_raw<double, ST>(raw, Dt, bt, D);
+      const int cn[4] = {raw.c.x, raw.c.y, raw.c.z, raw.c.w};
+      const float *p0 = sdir + ((int64_t)cn[0] * k + dir) * 3;
+      double dd[3][3];
+#pragma unroll
+      for (int kk = 0; kk < 3; ++kk) {
+        const float *pk = sdir + ((int64_t)cn[kk + 1] * k + dir) * 3;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) dd[kk][x] = (double)pk[x] - (double)p0[x];
+      }
+      double G[9];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          G[3 * r + c] = dd[0][r] * bt[0][c] + dd[1][r] * bt[1][c] + dd[2][r] * bt[2][c];
+      // eps^T D eps = (D0 + 2 D1) (tr e)^2 / 3 + (D0 - D1) |dev e|^2 + D2 |gamma|^2
+      // (the stress of fem_compute_ke.h:78-99), as a sum of six squares
+      const double e0 = G[0], e1 = G[4], e2 = G[8];
+      const double a = sqrt(cK * (D[0] + 2.0 * D[1]) / 3.0), b = sqrt(cK * (D[0] - D[1]));
+      const double c2 = sqrt(cK * D[2]);
+      e[0] = a * (e0 + e1 + e2);
+      e[1] = b * (e0 - e1) * 0.70710678118654752440;
+      e[2] = b * (e0 + e1 - 2.0 * e2) * 0.40824829046386301637;
+      e[3] = c2 * (G[1] + G[3]);
+      e[4] = c2 * (G[2] + G[6]);
+      e[5] = c2 * (G[5] + G[7]);
+    }
+#pragma unroll
+    for (int m = 0; m < 6; ++m) C = __builtin_amdgcn_mfma_f64_16x16x4f64(e[m], e[m], C, 0, 0, 0);
+  }
+  // mass part (rows sqrt(cM m) d_x) and D^T r_plain over owned free nodes
+  double gacc = 0.0;
+  for (int base = 4 * wave; base < no; base += 4 * W) {
+    const int l = base + qs;
+    double e[3] = {0.0, 0.0, 0.0};
+    if (l < no && dir < k) {
+      const int64_t g = ti.node_begin + l;
+      if (!A.fixed[g]) {
+        const double sq = sqrt(cM * A.mass[g]);
+        const float *d = sdir + ((int64_t)l * k + dir) * 3;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          e[x] = sq * (double)d[x];
+          gacc += (double)d[x] * A.r[3 * g + x];
+        }
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) C = __builtin_amdgcn_mfma_f64_16x16x4f64(e[m], e[m], C, 0, 0, 0);
+  }
+  // v_mfma_f64_16x16x4f64 leaves row lane / 16 + 4 v, column lane % 16 of
+  // the 16 x 16 sum in C[v] (checked on the device)
+#pragma unroll
+  for (int v = 0; v < 4; ++v) cred[wave][(qs + 4 * v) * 16 + dir] = C[v];
+  gacc += __shfl_xor(gacc, 16, 64);
+  gacc += __shfl_xor(gacc, 32, 64);
+  if (qs == 0) gred[wave][dir] = gacc;
+  __syncthreads();
+  for (int e = threadIdx.x; e < kSubPart; e += kSubGramBlock) {
+    double s = 0.0;
+    if (e < kSubPairs) {
+      // pair (i, j), i <= j, at e = j (j + 1) / 2 + i
+      int j = 0;
+      while ((j + 1) * (j + 2) / 2 <= e) ++j;
+      const int i = e - j * (j + 1) / 2;
+      for (int w = 0; w < W; ++w) s += cred[w][i * 16 + j];
+    } else {
+      for (int w = 0; w < W; ++w) s += gred[w][e - kSubPairs];
+    }
+    A.sub_part[(int64_t)tile * kSubPart + e] = s;
+  }
+}
+
+// Coefficients c of the start: the upper triangle of G = D^T A D and
+// g = D^T r_plain (packed as in the partial records), scaled to unit
+// diagonal, Cholesky with directions dropped whose pivot falls below 1e-12
+// (numerically dependent on the earlier ones, or zero); dropped directions
+// get c = 0.
+__device__ void sub_solve(const double *gram, int k, double *c) {
+  double L[kSubMax][kSubMax], s[kSubMax], y[kSubMax];
+  bool keep[kSubMax];
+  for (int i = 0; i < k; ++i) {
+    const double gii = gram[sub_pair(i, i)];
+    keep[i] = gii > 0.0 && isfinite(gii);
+    s[i] = keep[i] ? 1.0 / sqrt(gii) : 0.0;
+  }
+  for (int j = 0; j < k; ++j) {
+    if (!keep[j]) continue;
+    double dj = 1.0;
+    for (int m = 0; m < j; ++m)
+      if (keep[m]) dj -= L[j][m] * L[j][m];
+    if (!(dj > 1e-12)) {
+      keep[j] = false;
+      continue;
+    }
+    L[j][j] = sqrt(dj);
+    for (int i = j + 1; i < k; ++i) {
+      if (!keep[i]) continue;
+      double v = gram[sub_pair(j, i)] * s[i] * s[j];
+      for (int m = 0; m < j; ++m)
+        if (keep[m]) v -= L[i][m] * L[j][m];
+      L[i][j] = v / L[j][j];
+    }
+  }
+  for (int j = 0; j < k; ++j) {
+    if (!keep[j]) continue;
+    double v = gram[kSubPairs + j] * s[j];
+    for (int m = 0; m < j; ++m)
+      if (keep[m]) v -= L[j][m] * y[m];
+    y[j] = v / L[j][j];
+  }
+  bool ok = true;
+  for (int j = k - 1; j >= 0; --j) {
+    if (!keep[j]) {
+      c[j] = 0.0;
+      continue;
+    }
+    double v = y[j];
+    for (int i = j + 1; i < k; ++i)
+      if (keep[i]) v -= L[i][j] * c[i];
+    c[j] = v / L[j][j];
+    ok = ok && isfinite(c[j]);
+  }
+  for (int j = 0; j < k; ++j) c[j] = ok && keep[j] ? c[j] * s[j] : 0.0;
+  for (int j = k; j < kSubMax; ++j) c[j] = 0.0;
+}
+
+// One thread per body: its tiles' Gram partials summed in tile order, then
+// the coefficients into sub_coef[body].
+__global__ __launch_bounds__(64) void k_sub_solve(StepArgs A) {
+  const int b = blockIdx.x * 64 + threadIdx.x;
+  if (b >= A.B) return;
+  const SubState q = sub_state(A);
+  double gram[kSubPart];
+  for (int e = 0; e < kSubPart; ++e) gram[e] = 0.0;
+  for (int t = A.body_tiles[b]; t < A.body_tiles[b + 1]; ++t)
+    for (int e = 0; e < kSubPart; ++e) gram[e] += A.sub_part[(int64_t)t * kSubPart + e];
+  double c[kSubMax];
+  sub_solve(gram, q.k, c);
+  for (int j = 0; j < kSubMax; ++j) A.sub_coef[(int64_t)b * kSubMax + j] = c[j];
+}
+
+template <int BLOCK, typename AT, typename ST>
+__global__ __launch_bounds__(BLOCK) void k_sub_start(StepArgs A) {
+  extern __shared__ double lds[];
+  __shared__ double red[BLOCK / 64 + 1];
+  constexpr bool kOrdered = std::is_same<AT, Ordered>::value;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const TileInfo ti = A.tiles[tile];
+  const SubState q = sub_state(A);
+  const int k = q.k;
+  const int nn = ti.n_owned + ti.n_halo, no = ti.n_owned;
+  const double *vcur = A.vbuf + (int64_t)(1 - q.par) * 3 * A.N;  // v^n
+  const double *cf = A.sub_coef + (int64_t)ti.body * kSubMax;
+  double *sp = lds;         // w = D c at every tile node
+  double *sy = sp + 3 * nn; // K' w at owned nodes
+  for (int l = threadIdx.x; l < nn; l += BLOCK) {
+    const int64_t g = l < no ? ti.node_begin + l : A.halo[ti.halo_begin + l - no];
+    double wv[3] = {0.0, 0.0, 0.0};
+    if (!A.fixed[g]) {
+      const double vn[3] = {vcur[3 * g], vcur[3 * g + 1], vcur[3 * g + 2]};
+      float d1[3] = {0.f, 0.f, 0.f};
+      if (q.nh >= 1) {
+        const float4 h = A.vring[(int64_t)q.head * A.N + g];
+        d1[0] = h.x;
+        d1[1] = h.y;
+        d1[2] = h.z;
+      }
+      const double c0 = cf[0];
+#pragma unroll
+      for (int x = 0; x < 3; ++x) wv[x] = c0 * vn[x];
+      for (int j = 1; j < k; ++j) {
+        float d[3];
+        sub_dir(A, g, q, j, d1, vn, d);
+        const double cj = cf[j];
+#pragma unroll
+        for (int x = 0; x < 3; ++x) wv[x] += cj * (double)d[x];
+      }
+    }
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      sp[3 * l + x] = wv[x];
+      if (l < no) sy[3 * l + x] = 0.0;
+    }
+  }
+  __syncthreads();
+  double Dt[3] = {0.0, 0.0, 0.0};
+  if (std::is_same<ST, Folded>::value) {
+    const double4 td = A.tile_d[tile];
+    Dt[0] = td.x;
+    Dt[1] = td.y;
+    Dt[2] = td.z;
+  }
+  const int64_t S = A.S;
+  for (int s = ti.slot_begin + (int)threadIdx.x; s < ti.slot_begin + ti.n_slots; s += BLOCK) {
+    SlotRaw<ST> raw;
+    load_raw<ST>(A, S, s, raw);
+    double bt[3][3], D[3];
+    unpack_raw<double, ST>(raw, Dt, bt, D);
+    if constexpr (kOrdered)
+      slot_spmv_store<double>(raw.c, bt, D, sp, A.ordered_terms + 12 * (int64_t)s);
+    else
+      slot_spmv<double, double>(raw.c, bt, D, sp, sy, no);
+  }
+  __syncthreads();
+  double rr = 0.0;
+  for (int l = threadIdx.x; l < no; l += BLOCK) {
+    const int64_t g = ti.node_begin + l;
+    if constexpr (kOrdered) {
+      double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+      for (int e = A.ord_begin[g]; e < A.ord_begin[g + 1]; ++e) {
+        const int32_t code = A.ord_ent[e];
+        const double *o = A.ordered_terms + 12 * (int64_t)(code >> 2) + 3 * (code & 3);
+        y0 += o[0];
+        y1 += o[1];
+        y2 += o[2];
+      }
+      sy[3 * l] = y0;
+      sy[3 * l + 1] = y1;
+      sy[3 * l + 2] = y2;
+    }
+    if (A.fixed[g]) continue;
+    const double m = A.mass[g];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      const double wv = sp[3 * l + x];
+      const double r = A.r[3 * g + x] - (A.cM * m * wv + A.cK * sy[3 * l + x]);
+      A.r[3 * g + x] = r;
+      A.v[3 * g + x] += wv;
+      rr += r * r;
+    }
+  }
+  rr = block_sum<BLOCK>(rr, red);
+  if (threadIdx.x == 0) A.part_rr[tile] = rr;
+}
+
+// The history moves on: the slot just written (if any) becomes the newest,
+// v^n becomes v^(n-1).
+__global__ void k_sub_advance(StepArgs A) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (A.sub_state[1] >= 1) A.sub_state[0] = (A.sub_state[0] + 1) % A.sub_ring;
+    A.sub_state[1] = min(A.sub_state[1] + 1, A.sub_ring + 1);
+    A.sub_state[2] = 1 - A.sub_state[2];
+  }
+}
+
+template <typename AT, typename ST>
+hipError_t enqueue_sub(const StepArgs &a, size_t lds_gram, size_t lds_start, hipStream_t s) {
+  hipLaunchKernelGGL(k_sub_gram<ST>, dim3(a.ntiles), dim3(kSubGramBlock), lds_gram, s, a);
+  hipLaunchKernelGGL(k_sub_solve, dim3((a.B + 63) / 64), dim3(64), 0, s, a);
+  hipLaunchKernelGGL((k_sub_start<kSubStartBlock, AT, ST>), dim3(a.ntiles), dim3(kSubStartBlock),
+                     lds_start, s, a);
+  hipLaunchKernelGGL(k_sub_advance, dim3(1), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+size_t sub_gram_lds_bytes(int max_nodes, int directions) {
+  return sizeof(float) * 3 * (size_t)directions * (size_t)max_nodes;
+}
+
+hipError_t enqueue_subspace(const StepArgs &a, size_t lds_gram, size_t lds_start, hipStream_t s) {
+  if (a.sub_hist <= 0 || a.ntiles == 0) return hipSuccess;
+  if (a.ordered) {
+    if (a.store64) return enqueue_sub<Ordered, double>(a, lds_gram, lds_start, s);
+    if (a.folded) return enqueue_sub<Ordered, Folded>(a, lds_gram, lds_start, s);
+    return enqueue_sub<Ordered, float>(a, lds_gram, lds_start, s);
+  }
+  if (a.store64) return enqueue_sub<double, double>(a, lds_gram, lds_start, s);
+  if (a.folded) return enqueue_sub<double, Folded>(a, lds_gram, lds_start, s);
+  return enqueue_sub<double, float>(a, lds_gram, lds_start, s);
+}
+
 // Bodies still running after cg_max iterations complete their last update
 // here (v += alpha p) and record their stop; then x += dt v
 // (fem_position_update.h:33-38).

@@ -49,6 +49,7 @@ class Params(C.Structure):
         ("rotation_mode", C.c_int),
         ("warm_start", C.c_int),
         ("store_rotations", C.c_int),
+        ("subspace", C.c_int),
     ]
 
 
@@ -140,6 +141,8 @@ SIGNATURES = {
     "otfem_synchronize": (C.c_int, [_H, C.POINTER(Stats)]),
     "otfem_set_timing": (C.c_int, [_H, C.c_int]),
     "otfem_get_timing": (C.c_int, [_H, _dp, _dp, _dp, C.POINTER(C.c_longlong)]),
+    "otfem_get_timing_detail": (C.c_int, [_H, _dp, C.POINTER(C.c_longlong)]),
+    "otfem_subspace_coefficients": (C.c_int, [_H, _dp]),
     "otfem_iteration_totals": (C.c_int, [_H, C.POINTER(C.c_int64), C.c_int]),
     "otfem_unconverged_totals": (C.c_int, [_H, C.POINTER(C.c_int64), C.c_int]),
     "otfem_set_kernel_config": (C.c_int, [_H, C.c_int, C.c_int, C.c_int]),
@@ -220,10 +223,14 @@ def reference_params() -> Params:
     return default_params(0)
 
 
-def config_params(alpha=0.1, beta=0.001, tol=1e-6, cap=200, rotation_mode=0, store_rotations=0) -> Params:
+def config_params(alpha=0.1, beta=0.001, tol=1e-6, cap=200, rotation_mode=0, store_rotations=0,
+                  subspace=0) -> Params:
+    """Relative-residual settings.  `subspace` (0..7) turns on the subspace
+    CG start over that many past velocities (0: start from v^n)."""
     p = default_params(1)
     p.alpha, p.beta, p.tol, p.cg_max_iterations = alpha, beta, tol, cap
     p.rotation_mode, p.store_rotations = rotation_mode, store_rotations
+    p.subspace = subspace
     return p
 
 
@@ -414,6 +421,20 @@ class Solver:
         n = C.c_longlong()
         self._check(self.lib.otfem_get_timing(self.h, C.byref(e), C.byref(c), C.byref(w), C.byref(n)))
         return e.value, c.value, w.value, n.value
+
+    def subspace_coefficients(self):
+        """[B, 12] coefficients of the last subspace start (v^n's first, then
+        the differences from the newest, then the network's direction)."""
+        c = np.zeros((self.B, 12))
+        self._check(self.lib.otfem_subspace_coefficients(self.h, _dptr(c)))
+        return c
+
+    def timing_detail(self):
+        """dict of accumulated ms (network, element, subspace start, CG) and steps."""
+        ms = np.zeros(4)
+        n = C.c_longlong()
+        self._check(self.lib.otfem_get_timing_detail(self.h, _dptr(ms), C.byref(n)))
+        return dict(network=ms[0], element=ms[1], subspace=ms[2], cg=ms[3], steps=n.value)
 
     def iteration_totals(self, reset=False):
         tot = np.empty(self.B, dtype=np.int64)
