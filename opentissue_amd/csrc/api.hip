@@ -110,6 +110,8 @@ struct otfem_solver {
   double *sub_part = nullptr;
   int32_t *body_tiles = nullptr;  // [B+1]
   double *sub_coef = nullptr;     // [B][kSubMax]
+  int32_t *prim_begin = nullptr;  // [ntiles+1] primary slots per tile
+  int32_t *prim_slot = nullptr;   // [T]
   size_t lds_sub = 0;
   hipGraphExec_t g_sub = nullptr;
 };
@@ -193,7 +195,7 @@ void free_device(otfem_solver *h) {
   dfree(h->ws_bad); dfree(h->act);
   dfree(h->ord_begin); dfree(h->ord_ent); dfree(h->ordered_terms);
   dfree(h->vring); dfree(h->vbuf); dfree(h->sub_state); dfree(h->sub_part);
-  dfree(h->body_tiles); dfree(h->sub_coef);
+  dfree(h->body_tiles); dfree(h->sub_coef); dfree(h->prim_begin); dfree(h->prim_slot);
   h->ring_slots = 0;
   drop_graphs(h);
   if (h->done_host) (void)hipHostFree(h->done_host);
@@ -550,7 +552,10 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
   h->lds_cg = cg_lds_bytes(L.max_tile_nodes, L.max_tile_owned);
   hipDeviceProp_t prop;
   HIPCHK(h, hipGetDeviceProperties(&prop, h->device));
-  if (h->lds_elem > (size_t)prop.sharedMemPerBlock || h->lds_cg > (size_t)prop.sharedMemPerBlock)
+  // the kernels' static LDS (block reductions) is under 1 KiB
+  constexpr size_t kStaticLds = 1024;
+  if (h->lds_elem + kStaticLds > (size_t)prop.sharedMemPerBlock ||
+      h->lds_cg + kStaticLds > (size_t)prop.sharedMemPerBlock)
     return fail(h, OTFEM_INVALID_ARG, "tile too large for LDS; lower target_tile_nodes");
   // material data changed: captured graphs are rebuilt on the next step
   drop_graphs(h);
@@ -606,6 +611,8 @@ StepArgs make_args(otfem_solver *h, double dt, int warp, const otfem_params &P) 
   a.sub_part = h->sub_part;
   a.body_tiles = h->body_tiles;
   a.sub_coef = h->sub_coef;
+  a.prim_begin = h->prim_begin;
+  a.prim_slot = h->prim_slot;
   a.sub_net = 0;
   a.vnet = nullptr;
   return a;
@@ -631,6 +638,25 @@ otfem_status sub_ensure(otfem_solver *h, int hist, int directions) {
     HIPCHK(h, hipMemsetAsync(h->sub_state, 0, 3 * sizeof(int32_t), h->stream));
   }
   if (!h->vbuf) HIPCHK(h, dalloc(&h->vbuf, 6 * N));
+  if (!h->prim_begin) {
+    // every tile's primary slots (one per tet), in slot order
+    std::vector<int32_t> pb(nt + 1, 0), ps;
+    ps.reserve(h->L.T);
+    for (int64_t i = 0; i < nt; ++i) {
+      const TileInfo &ti = h->L.tiles[i];
+      for (int32_t sl = ti.slot_begin; sl < ti.slot_begin + ti.n_slots; ++sl)
+        if (h->L.slot_tet[sl] < 0) ps.push_back(sl);
+      pb[i + 1] = (int32_t)ps.size();
+    }
+    HIPCHK(h, dalloc(&h->prim_begin, nt + 1));
+    HIPCHK(h, dalloc(&h->prim_slot, std::max<size_t>(ps.size(), 1)));
+    HIPCHK(h, hipMemcpyAsync(h->prim_begin, pb.data(), (nt + 1) * sizeof(int32_t), hipMemcpyHostToDevice,
+                             h->stream));
+    if (!ps.empty())
+      HIPCHK(h, hipMemcpyAsync(h->prim_slot, ps.data(), ps.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                               h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
   if (h->ring_slots < hist) {
     // steps enqueued earlier may still read the old ring
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -641,10 +667,11 @@ otfem_status sub_ensure(otfem_solver *h, int hist, int directions) {
     h->ring_slots = hist;
     HIPCHK(h, hipMemsetAsync(h->sub_state, 0, 3 * sizeof(int32_t), h->stream));
   }
-  const size_t lds = sub_gram_lds_bytes(h->L.max_tile_nodes, directions);
+  const size_t lds = sub_gram_lds_bytes(h->L.max_tile_nodes, h->L.max_tile_owned, directions);
   hipDeviceProp_t prop;
   HIPCHK(h, hipGetDeviceProperties(&prop, h->device));
-  if (lds > (size_t)prop.sharedMemPerBlock)
+  // a launch needs its dynamic and static LDS within the workgroup's share
+  if (lds + sub_gram_static_lds_bytes() > (size_t)prop.sharedMemPerBlock)
     return fail(h, OTFEM_INVALID_ARG,
                 "tile too large for the subspace start's LDS; use fewer directions or lower target_tile_nodes");
   h->lds_sub = std::max(h->lds_sub, lds);
@@ -1410,6 +1437,19 @@ static otfem_status autotune_layout_body(const otfem_mesh_desc *desc, const doub
     d.target_tile_nodes = c;
     otfem_handle h = nullptr;
     otfem_status st = otfem_create(&d, &h);
+    if (st == OTFEM_OK && params && params->subspace > 0) {
+      // a layout whose tiles cannot hold the subspace start's directions in
+      // LDS cannot run these settings
+      int dev = 0;
+      hipDeviceProp_t prop;
+      if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+          sub_gram_lds_bytes(h->L.max_tile_nodes, h->L.max_tile_owned, 2 + params->subspace) +
+                  sub_gram_static_lds_bytes() > (size_t)prop.sharedMemPerBlock) {
+        why += "target " + std::to_string(c) + ": tiles too large for the subspace start; ";
+        otfem_destroy(h);
+        continue;
+      }
+    }
     if (st == OTFEM_OK) st = otfem_init_materials(h, young, poisson, density, 0.0, 0.0, 0.0);
     if (st == OTFEM_OK && f_ext) st = otfem_set_external_forces(h, f_ext);
     int b = 0;

@@ -109,6 +109,8 @@ struct StepArgs {
   double *sub_part;          // [ntiles][kSubPart] per-tile Gram partials
   const int32_t *body_tiles; // [B+1] body b owns tiles [body_tiles[b], body_tiles[b+1])
   double *sub_coef;          // [B][kSubMax] the step's coefficients c per body
+  const int32_t *prim_begin; // [ntiles+1] tile t's primary slots are prim_slot[prim_begin[t] ..]
+  const int32_t *prim_slot;  // [T] slot of each tet's primary copy, grouped by tile
 };
 
 // Subspace start: at most kSubMax directions; a tile's partial record holds
@@ -139,7 +141,9 @@ hipError_t enqueue_finish(const StepArgs &a, const LaunchConfig &lc, hipStream_t
 // partials of r.r; then the ring advances.  lds_gram / lds_start from
 // sub_gram_lds_bytes / cg_lds_bytes.
 hipError_t enqueue_subspace(const StepArgs &a, size_t lds_gram, size_t lds_start, hipStream_t s);
-size_t sub_gram_lds_bytes(int max_nodes, int directions);
+size_t sub_gram_lds_bytes(int max_nodes, int max_owned, int directions);
+// Static LDS of k_sub_gram (a launch needs dynamic + static within the CU's LDS).
+size_t sub_gram_static_lds_bytes();
 // Surface coupling: out[3i..] = sum_k w[i].k x[3 ids[i].k ..] for bound
 // vertices (ids[i].x >= 0, internal node ids), rest[3i..] for the others.
 hipError_t enqueue_surface_update(int64_t n, const int4 *ids, const double4 *w, const double *rest,

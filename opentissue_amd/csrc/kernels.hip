@@ -40,6 +40,7 @@
 // sums never leave a workgroup.
 #include "kernels.h"
 
+#include <algorithm>
 #include <cmath>
 #include <type_traits>
 
@@ -1079,8 +1080,10 @@ void k_cg_iter(StepArgs A, int it) {
 
 namespace {
 
-constexpr int kSubGramBlock = 256;
+constexpr int kSubGramBlock = 768;
 constexpr int kSubStartBlock = 256;
+constexpr int kSubSolveBlock = 1024;
+constexpr int kSubRounds = 4;  // Gram rounds whose slot records are loaded ahead
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int sub_pair(int i, int j) { return j * (j + 1) / 2 + i; }
@@ -1123,69 +1126,96 @@ __device__ __forceinline__ void sub_dir(const StepArgs &A, int64_t g, const SubS
   }
 }
 
+// Scale factors of E~ = L eps with L^T L = cK D: eps^T D eps =
+// (D0 + 2 D1) (tr e)^2 / 3 + (D0 - D1) |dev e|^2 + D2 |gamma|^2 (the stress
+// of fem_compute_ke.h:78-99) as a sum of six squares.
+__device__ __forceinline__ void sub_factors(const double D[3], double cK, double f[3]) {
+  f[0] = sqrt(cK * (D[0] + 2.0 * D[1]) / 3.0);
+  f[1] = sqrt(cK * (D[0] - D[1]));
+  f[2] = sqrt(cK * D[2]);
+}
+
 // Per-tile partials of D^T A D (A = cM M + cK K') and D^T r_plain.  Owned
 // nodes also store v^n (FP64) and v^n - v^(n-1) (FP32, ring slot `head`); no
 // kernel of this step reads either before the kernel boundary.
 template <typename ST>
 __global__ __launch_bounds__(kSubGramBlock) void k_sub_gram(StepArgs A) {
-  extern __shared__ float sdir[];  // [nn][k][3]
+  // LDS: directions [nn][k][3] (FP32); after the passes, the waves' 16 x 16
+  // sums
+  extern __shared__ double lds_g[];
   constexpr int W = kSubGramBlock / 64;
-  __shared__ double cred[W][256];
   __shared__ double gred[W][16];
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const TileInfo ti = A.tiles[tile];
   const SubState q = sub_state(A);
   const int k = q.k;
   const int nn = ti.n_owned + ti.n_halo, no = ti.n_owned;
+  float *sdir = reinterpret_cast<float *>(lds_g);
   const double *vprev = A.vbuf + (int64_t)q.par * 3 * A.N;
   double *vcur = A.vbuf + (int64_t)(1 - q.par) * 3 * A.N;
+  const double cK = A.cK, cM = A.cM;
   for (int l = threadIdx.x; l < nn; l += kSubGramBlock) {
     const int64_t g = l < no ? ti.node_begin + l : A.halo[ti.halo_begin + l - no];
     const double vn[3] = {A.v[3 * g], A.v[3 * g + 1], A.v[3 * g + 2]};
+    const bool fx = A.fixed[g] != 0;
     float d1[3] = {0.f, 0.f, 0.f};
     if (q.prev >= 1)
 #pragma unroll
       for (int x = 0; x < 3; ++x) d1[x] = (float)(vn[x] - vprev[3 * g + x]);
+    // every direction's loads are issued before any is stored
+    float d[kSubMax][3];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) d[0][x] = (float)vn[x];
+#pragma unroll
+    for (int j = 1; j < kSubMax; ++j)
+      if (j < k) sub_dir(A, g, q, j, d1, vn, d[j]);
     if (l < no) {
 #pragma unroll
       for (int x = 0; x < 3; ++x) vcur[3 * g + x] = vn[x];
       if (q.prev >= 1) A.vring[(int64_t)q.head * A.N + g] = make_float4(d1[0], d1[1], d1[2], 0.f);
     }
-    const bool fx = A.fixed[g] != 0;
     float *o = sdir + (int64_t)l * k * 3;
 #pragma unroll
-    for (int x = 0; x < 3; ++x) o[x] = fx ? 0.f : (float)vn[x];
-    for (int j = 1; j < k; ++j) {
-      float d[3];
-      sub_dir(A, g, q, j, d1, vn, d);
+    for (int j = 0; j < kSubMax; ++j)
+      if (j < k)
 #pragma unroll
-      for (int x = 0; x < 3; ++x) o[3 * j + x] = fx ? 0.f : d[x];
-    }
+        for (int x = 0; x < 3; ++x) o[3 * j + x] = fx ? 0.f : d[j][x];
   }
   __syncthreads();
 
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int qs = lane >> 4, dir = lane & 15;  // lane = (slot of the round, direction)
-  v4d C = {0.0, 0.0, 0.0, 0.0};
-  double Dt[3] = {0.0, 0.0, 0.0};
+  const int qs = lane >> 4, dir = lane & 15;  // lane = (tet of the round, direction)
+  const bool live = dir < k;
+  v4d C0 = {0.0, 0.0, 0.0, 0.0}, C1 = {0.0, 0.0, 0.0, 0.0};
+  double Dt[3] = {0.0, 0.0, 0.0}, ft[3] = {0.0, 0.0, 0.0};
   if (std::is_same<ST, Folded>::value) {
     const double4 td = A.tile_d[tile];
     Dt[0] = td.x;
     Dt[1] = td.y;
     Dt[2] = td.z;
+    sub_factors(Dt, cK, ft);
   }
   const int64_t S = A.S;
-  const double cK = A.cK, cM = A.cM;
-  const int send = ti.slot_begin + ti.n_slots;
-  // element part: E~ rows of 4 slots per wave and round (wave-uniform loop)
-  for (int base = ti.slot_begin + 4 * wave; base < send; base += 4 * W) {
-    const int s = base + qs;
+  // element part over the tile's primary slots (each tet once): E~ rows of 4
+  // tets per wave and round, the next round's slot record in flight while
+  // this one is used (wave-uniform loop)
+  const int pb = A.prim_begin[tile], pe = A.prim_begin[tile + 1];
+  auto fetch = [&](int i, SlotRaw<ST> &raw, bool &valid) {
+    valid = i < pe && live;
+    load_raw<ST>(A, S, A.prim_slot[min(i, pe - 1)], raw);
+  };
+  auto rows = [&](const SlotRaw<ST> &raw, bool valid) {
     double e[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (s < send && dir < k && A.slot_tet[s] < 0) {  // each tet once: its primary copy
-      SlotRaw<ST> raw;
-      load_raw<ST>(A, S, s, raw);
-      double bt[3][3], D[3];
+    if (valid) {
+      double bt[3][3], D[3], f[3];
       unpack_raw<double, ST>(raw, Dt, bt, D);
+      if (std::is_same<ST, Folded>::value) {
+        f[0] = ft[0];
+        f[1] = ft[1];
+        f[2] = ft[2];
+      } else {
+        sub_factors(D, cK, f);
+      }
       const int cn[4] = {raw.c.x, raw.c.y, raw.c.z, raw.c.w};
       const float *p0 = sdir + ((int64_t)cn[0] * k + dir) * 3;
       double dd[3][3];
@@ -1201,27 +1231,51 @@ __global__ __launch_bounds__(kSubGramBlock) void k_sub_gram(StepArgs A) {
 #pragma unroll
         for (int c = 0; c < 3; ++c)
           G[3 * r + c] = dd[0][r] * bt[0][c] + dd[1][r] * bt[1][c] + dd[2][r] * bt[2][c];
-      // eps^T D eps = (D0 + 2 D1) (tr e)^2 / 3 + (D0 - D1) |dev e|^2 + D2 |gamma|^2
-      // (the stress of fem_compute_ke.h:78-99), as a sum of six squares
       const double e0 = G[0], e1 = G[4], e2 = G[8];
-      const double a = sqrt(cK * (D[0] + 2.0 * D[1]) / 3.0), b = sqrt(cK * (D[0] - D[1]));
-      const double c2 = sqrt(cK * D[2]);
-      e[0] = a * (e0 + e1 + e2);
-      e[1] = b * (e0 - e1) * 0.70710678118654752440;
-      e[2] = b * (e0 + e1 - 2.0 * e2) * 0.40824829046386301637;
-      e[3] = c2 * (G[1] + G[3]);
-      e[4] = c2 * (G[2] + G[6]);
-      e[5] = c2 * (G[5] + G[7]);
+      e[0] = f[0] * (e0 + e1 + e2);
+      e[1] = f[1] * (e0 - e1) * 0.70710678118654752440;
+      e[2] = f[1] * (e0 + e1 - 2.0 * e2) * 0.40824829046386301637;
+      e[3] = f[2] * (G[1] + G[3]);
+      e[4] = f[2] * (G[2] + G[6]);
+      e[5] = f[2] * (G[5] + G[7]);
     }
+    // two accumulators: consecutive MFMAs do not wait for each other
+    C0 = __builtin_amdgcn_mfma_f64_16x16x4f64(e[0], e[0], C0, 0, 0, 0);
+    C1 = __builtin_amdgcn_mfma_f64_16x16x4f64(e[1], e[1], C1, 0, 0, 0);
+    C0 = __builtin_amdgcn_mfma_f64_16x16x4f64(e[2], e[2], C0, 0, 0, 0);
+    C1 = __builtin_amdgcn_mfma_f64_16x16x4f64(e[3], e[3], C1, 0, 0, 0);
+    C0 = __builtin_amdgcn_mfma_f64_16x16x4f64(e[4], e[4], C0, 0, 0, 0);
+    C1 = __builtin_amdgcn_mfma_f64_16x16x4f64(e[5], e[5], C1, 0, 0, 0);
+  };
+  // rounds of 4 tets per wave; kSubRounds rounds' slot records are in
+  // flight while the previous kSubRounds are used (wave-uniform loop)
+  constexpr int U = kSubRounds;
+  constexpr int R = 4 * W;  // tets per round over the workgroup
+  int base = pb + 4 * wave + qs;
+  if (base - qs < pe) {
+    SlotRaw<ST> ra[U], rb[U];
+    bool va[U], vb[U];
 #pragma unroll
-    for (int m = 0; m < 6; ++m) C = __builtin_amdgcn_mfma_f64_16x16x4f64(e[m], e[m], C, 0, 0, 0);
+    for (int u = 0; u < U; ++u) fetch(base + u * R, ra[u], va[u]);
+    for (;; base += 2 * U * R) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) fetch(base + (U + u) * R, rb[u], vb[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) rows(ra[u], va[u]);
+      if (base - qs + U * R >= pe) break;
+#pragma unroll
+      for (int u = 0; u < U; ++u) fetch(base + (2 * U + u) * R, ra[u], va[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) rows(rb[u], vb[u]);
+      if (base - qs + 2 * U * R >= pe) break;
+    }
   }
   // mass part (rows sqrt(cM m) d_x) and D^T r_plain over owned free nodes
   double gacc = 0.0;
-  for (int base = 4 * wave; base < no; base += 4 * W) {
-    const int l = base + qs;
+  for (int nb = 4 * wave; nb < no; nb += 4 * W) {
+    const int l = nb + qs;
     double e[3] = {0.0, 0.0, 0.0};
-    if (l < no && dir < k) {
+    if (l < no && live) {
       const int64_t g = ti.node_begin + l;
       if (!A.fixed[g]) {
         const double sq = sqrt(cM * A.mass[g]);
@@ -1233,16 +1287,19 @@ __global__ __launch_bounds__(kSubGramBlock) void k_sub_gram(StepArgs A) {
         }
       }
     }
-#pragma unroll
-    for (int m = 0; m < 3; ++m) C = __builtin_amdgcn_mfma_f64_16x16x4f64(e[m], e[m], C, 0, 0, 0);
+    C0 = __builtin_amdgcn_mfma_f64_16x16x4f64(e[0], e[0], C0, 0, 0, 0);
+    C1 = __builtin_amdgcn_mfma_f64_16x16x4f64(e[1], e[1], C1, 0, 0, 0);
+    C0 = __builtin_amdgcn_mfma_f64_16x16x4f64(e[2], e[2], C0, 0, 0, 0);
   }
-  // v_mfma_f64_16x16x4f64 leaves row lane / 16 + 4 v, column lane % 16 of
-  // the 16 x 16 sum in C[v] (checked on the device)
-#pragma unroll
-  for (int v = 0; v < 4; ++v) cred[wave][(qs + 4 * v) * 16 + dir] = C[v];
   gacc += __shfl_xor(gacc, 16, 64);
   gacc += __shfl_xor(gacc, 32, 64);
   if (qs == 0) gred[wave][dir] = gacc;
+  __syncthreads();  // every wave is done with the staged data: reuse its LDS
+  double *cred = lds_g;  // [W][256]
+  // v_mfma_f64_16x16x4f64 leaves row lane / 16 + 4 v, column lane % 16 of
+  // the 16 x 16 sum in C[v] (checked on the device)
+#pragma unroll
+  for (int v = 0; v < 4; ++v) cred[wave * 256 + (qs + 4 * v) * 16 + dir] = C0[v] + C1[v];
   __syncthreads();
   for (int e = threadIdx.x; e < kSubPart; e += kSubGramBlock) {
     double s = 0.0;
@@ -1251,7 +1308,7 @@ __global__ __launch_bounds__(kSubGramBlock) void k_sub_gram(StepArgs A) {
       int j = 0;
       while ((j + 1) * (j + 2) / 2 <= e) ++j;
       const int i = e - j * (j + 1) / 2;
-      for (int w = 0; w < W; ++w) s += cred[w][i * 16 + j];
+      for (int w = 0; w < W; ++w) s += cred[w * 256 + i * 16 + j];
     } else {
       for (int w = 0; w < W; ++w) s += gred[w][e - kSubPairs];
     }
@@ -1259,73 +1316,105 @@ __global__ __launch_bounds__(kSubGramBlock) void k_sub_gram(StepArgs A) {
   }
 }
 
-// Coefficients c of the start: the upper triangle of G = D^T A D and
-// g = D^T r_plain (packed as in the partial records), scaled to unit
-// diagonal, Cholesky with directions dropped whose pivot falls below 1e-12
-// (numerically dependent on the earlier ones, or zero); dropped directions
-// get c = 0.
-__device__ void sub_solve(const double *gram, int k, double *c) {
-  double L[kSubMax][kSubMax], s[kSubMax], y[kSubMax];
-  bool keep[kSubMax];
-  for (int i = 0; i < k; ++i) {
-    const double gii = gram[sub_pair(i, i)];
-    keep[i] = gii > 0.0 && isfinite(gii);
-    s[i] = keep[i] ? 1.0 / sqrt(gii) : 0.0;
+// Coefficients of the start on one wave: lane i holds row i of G = D^T A D
+// scaled to unit diagonal; Cholesky by columns with directions dropped whose
+// pivot falls below 1e-12 (numerically dependent on the earlier ones, or
+// zero); dropped directions get c = 0.  out[kSubMax] receives c.
+__device__ void sub_solve_wave(const double *gram, int k, double *out) {
+  const int lane = threadIdx.x & 63;
+  const bool in = lane < k;
+  const double gii = in ? gram[sub_pair(lane, lane)] : 0.0;
+  const bool kp = in && gii > 0.0 && isfinite(gii);
+  const double si = kp ? 1.0 / sqrt(gii) : 0.0;
+  double row[kSubMax];
+#pragma unroll
+  for (int j = 0; j < kSubMax; ++j) {
+    const double sj = __shfl(si, j, 64);
+    row[j] = (in && j < k) ? gram[sub_pair(min(lane, j), max(lane, j))] * si * sj : 0.0;
   }
-  for (int j = 0; j < k; ++j) {
-    if (!keep[j]) continue;
-    double dj = 1.0;
-    for (int m = 0; m < j; ++m)
-      if (keep[m]) dj -= L[j][m] * L[j][m];
-    if (!(dj > 1e-12)) {
-      keep[j] = false;
-      continue;
-    }
-    L[j][j] = sqrt(dj);
-    for (int i = j + 1; i < k; ++i) {
-      if (!keep[i]) continue;
-      double v = gram[sub_pair(j, i)] * s[i] * s[j];
+  const double bi = in ? gram[kSubPairs + lane] * si : 0.0;
+  unsigned keep = (unsigned)(__ballot(kp) & ((1ull << kSubMax) - 1));
+  double ldiag[kSubMax];
+#pragma unroll
+  for (int j = 0; j < kSubMax; ++j) {
+    ldiag[j] = 1.0;
+    if ((keep >> j) & 1u) {
+      double d = row[j];
+#pragma unroll
       for (int m = 0; m < j; ++m)
-        if (keep[m]) v -= L[i][m] * L[j][m];
-      L[i][j] = v / L[j][j];
+        if ((keep >> m) & 1u) d -= row[m] * row[m];
+      d = __shfl(d, j, 64);
+      if (!(d > 1e-12)) {
+        keep &= ~(1u << j);
+      } else {
+        const double ljj = sqrt(d);
+        ldiag[j] = ljj;
+        double v = row[j];
+#pragma unroll
+        for (int m = 0; m < j; ++m)
+          if ((keep >> m) & 1u) v -= row[m] * __shfl(row[m], j, 64);
+        if (lane > j)
+          row[j] = v / ljj;
+        else if (lane == j)
+          row[j] = ljj;
+      }
     }
   }
-  for (int j = 0; j < k; ++j) {
-    if (!keep[j]) continue;
-    double v = gram[kSubPairs + j] * s[j];
-    for (int m = 0; m < j; ++m)
-      if (keep[m]) v -= L[j][m] * y[m];
-    y[j] = v / L[j][j];
-  }
-  bool ok = true;
-  for (int j = k - 1; j >= 0; --j) {
-    if (!keep[j]) {
-      c[j] = 0.0;
-      continue;
+  double y[kSubMax];
+#pragma unroll
+  for (int j = 0; j < kSubMax; ++j) {
+    y[j] = 0.0;
+    if ((keep >> j) & 1u) {
+      double v = bi;
+#pragma unroll
+      for (int m = 0; m < j; ++m)
+        if ((keep >> m) & 1u) v -= row[m] * y[m];
+      y[j] = __shfl(v, j, 64) / ldiag[j];
     }
-    double v = y[j];
-    for (int i = j + 1; i < k; ++i)
-      if (keep[i]) v -= L[i][j] * c[i];
-    c[j] = v / L[j][j];
-    ok = ok && isfinite(c[j]);
   }
-  for (int j = 0; j < k; ++j) c[j] = ok && keep[j] ? c[j] * s[j] : 0.0;
-  for (int j = k; j < kSubMax; ++j) c[j] = 0.0;
+  const bool mine = lane < kSubMax && ((keep >> (lane < kSubMax ? lane : 0)) & 1u);
+  double cl = 0.0;
+#pragma unroll
+  for (int j = kSubMax - 1; j >= 0; --j) {
+    if ((keep >> j) & 1u) {
+      const double t = wave_sum((mine && lane > j) ? row[j] * cl : 0.0);
+      const double cj = (y[j] - t) / ldiag[j];
+      if (lane == j) cl = cj;
+    }
+  }
+  const bool ok = __ballot(!isfinite(cl)) == 0;
+  if (lane < kSubMax) out[lane] = (ok && mine) ? cl * si : 0.0;
 }
 
-// One thread per body: its tiles' Gram partials summed in tile order, then
-// the coefficients into sub_coef[body].
-__global__ __launch_bounds__(64) void k_sub_solve(StepArgs A) {
-  const int b = blockIdx.x * 64 + threadIdx.x;
-  if (b >= A.B) return;
+// One workgroup per body: its tiles' Gram partials summed in a fixed order
+// (wave w takes tiles w, w + 4, ..., then the waves in order), then the
+// coefficients into sub_coef[body].
+__global__ __launch_bounds__(kSubSolveBlock) void k_sub_solve(StepArgs A) {
+  constexpr int W = kSubSolveBlock / 64;
+  __shared__ double part[W][kSubPart];
+  __shared__ double gram[kSubPart];
+  const int b = blockIdx.x;
   const SubState q = sub_state(A);
-  double gram[kSubPart];
-  for (int e = 0; e < kSubPart; ++e) gram[e] = 0.0;
-  for (int t = A.body_tiles[b]; t < A.body_tiles[b + 1]; ++t)
-    for (int e = 0; e < kSubPart; ++e) gram[e] += A.sub_part[(int64_t)t * kSubPart + e];
-  double c[kSubMax];
-  sub_solve(gram, q.k, c);
-  for (int j = 0; j < kSubMax; ++j) A.sub_coef[(int64_t)b * kSubMax + j] = c[j];
+  const int t0 = A.body_tiles[b], t1 = A.body_tiles[b + 1];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double s0 = 0.0, s1 = 0.0;
+  const bool two = lane + 64 < kSubPart;
+#pragma unroll 4
+  for (int t = t0 + wave; t < t1; t += W) {
+    const double *p = A.sub_part + (int64_t)t * kSubPart;
+    s0 += p[lane];
+    if (two) s1 += p[lane + 64];
+  }
+  part[wave][lane] = s0;
+  if (two) part[wave][lane + 64] = s1;
+  __syncthreads();
+  for (int e = threadIdx.x; e < kSubPart; e += kSubSolveBlock) {
+    double s = 0.0;
+    for (int w = 0; w < W; ++w) s += part[w][e];
+    gram[e] = s;
+  }
+  __syncthreads();
+  if (wave == 0) sub_solve_wave(gram, q.k, A.sub_coef + (int64_t)b * kSubMax);
 }
 
 template <int BLOCK, typename AT, typename ST>
@@ -1354,15 +1443,21 @@ __global__ __launch_bounds__(BLOCK) void k_sub_start(StepArgs A) {
         d1[1] = h.y;
         d1[2] = h.z;
       }
+      // every direction's loads are issued before any is used
+      float d[kSubMax][3];
+#pragma unroll
+      for (int j = 1; j < kSubMax; ++j)
+        if (j < k) sub_dir(A, g, q, j, d1, vn, d[j]);
       const double c0 = cf[0];
 #pragma unroll
       for (int x = 0; x < 3; ++x) wv[x] = c0 * vn[x];
-      for (int j = 1; j < k; ++j) {
-        float d[3];
-        sub_dir(A, g, q, j, d1, vn, d);
-        const double cj = cf[j];
 #pragma unroll
-        for (int x = 0; x < 3; ++x) wv[x] += cj * (double)d[x];
+      for (int j = 1; j < kSubMax; ++j) {
+        if (j < k) {
+          const double cj = cf[j];
+#pragma unroll
+          for (int x = 0; x < 3; ++x) wv[x] += cj * (double)d[j][x];
+        }
       }
     }
 #pragma unroll
@@ -1435,7 +1530,7 @@ __global__ void k_sub_advance(StepArgs A) {
 template <typename AT, typename ST>
 hipError_t enqueue_sub(const StepArgs &a, size_t lds_gram, size_t lds_start, hipStream_t s) {
   hipLaunchKernelGGL(k_sub_gram<ST>, dim3(a.ntiles), dim3(kSubGramBlock), lds_gram, s, a);
-  hipLaunchKernelGGL(k_sub_solve, dim3((a.B + 63) / 64), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_sub_solve, dim3(a.B), dim3(kSubSolveBlock), 0, s, a);
   hipLaunchKernelGGL((k_sub_start<kSubStartBlock, AT, ST>), dim3(a.ntiles), dim3(kSubStartBlock),
                      lds_start, s, a);
   hipLaunchKernelGGL(k_sub_advance, dim3(1), dim3(64), 0, s, a);
@@ -1444,9 +1539,14 @@ hipError_t enqueue_sub(const StepArgs &a, size_t lds_gram, size_t lds_start, hip
 
 }  // namespace
 
-size_t sub_gram_lds_bytes(int max_nodes, int directions) {
-  return sizeof(float) * 3 * (size_t)directions * (size_t)max_nodes;
+size_t sub_gram_lds_bytes(int max_nodes, int max_owned, int directions) {
+  // the directions of every tile node, then (reused) the waves' 16 x 16 sums
+  (void)max_owned;
+  const size_t staged = sizeof(double) * ((3 * (size_t)max_nodes * directions + 1) / 2);
+  return std::max(staged, sizeof(double) * 256 * (kSubGramBlock / 64));
 }
+
+size_t sub_gram_static_lds_bytes() { return sizeof(double) * 16 * (kSubGramBlock / 64); }
 
 hipError_t enqueue_subspace(const StepArgs &a, size_t lds_gram, size_t lds_start, hipStream_t s) {
   if (a.sub_hist <= 0 || a.ntiles == 0) return hipSuccess;

@@ -209,3 +209,48 @@ def test_subspace_deterministic_bits():
             s.simulate(w.dt, True, P)
         out.append(s.state())
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_subspace_lds_limit_is_an_error_not_a_launch():
+    """A tiling whose tile node table with the requested directions exceeds
+    the workgroup's LDS (dynamic plus static) is rejected by the step with
+    OTFEM_INVALID_ARG before any launch; fewer directions still run."""
+    from opentissue_amd import fem, workloads
+    w = workloads.cantilever()
+    s = fem.Solver(w.coords, w.tets, w.fixed, target_tile_nodes=1500)
+    s.init_materials(w.young, w.poisson, w.density)
+    s.set_external_forces(w.forces)
+    assert s.layout_info()["max_tile_nodes"] > 1200
+    model = fem.Model(seed=0)
+    s.set_warmstart_model(model)
+    P = fem.config_params(cap=3000, subspace=10)
+    P.warm_start = fem.WARM_LEARNED
+    with pytest.raises(fem.OtfemError) as e:
+        s.simulate(w.dt, True, P)
+    assert e.value.status == fem.INVALID_ARG
+    P1 = fem.config_params(cap=3000, subspace=1)
+    for _ in range(3):
+        st = s.simulate(w.dt, True, P1)
+        assert st.bodies_converged == 1
+
+
+def test_subspace_learned_on_the_64_cube_tiles():
+    """The learned start with the most directions (10 differences and the
+    network's) on the 64^3 cube's default tiles, whose largest node table
+    (~980 nodes) is the tightest fit for the Gram kernel's LDS."""
+    from opentissue_amd import fem, workloads
+    w = workloads.cube64()
+    s = fem.Solver(w.coords, w.tets, w.fixed)
+    s.init_materials(w.young, w.poisson, w.density)
+    s.set_external_forces(w.forces)
+    model = fem.Model(seed=0)
+    s.set_warmstart_model(model)
+    P = fem.config_params(cap=5000, subspace=10)
+    P.warm_start = fem.WARM_LEARNED
+    its = []
+    for _ in range(14):
+        st = s.simulate(w.dt, True, P)
+        assert st.bodies_converged == 1 and st.flags == 0
+        its.append(st.iterations)
+    print("64^3 cube, learned start with 12 directions:", its)
+    assert np.mean(its[-4:]) < 0.8 * its[0]
