@@ -152,6 +152,23 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
  * and any error of the enqueued steps are reported by otfem_synchronize.  No
  * reference counterpart: fem::simulate is synchronous. */
 otfem_status otfem_synchronize(otfem_handle h, otfem_stats *stats);
+/* otfem_step without any allocation, for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid it).
+ * No reference counterpart.  It enqueues the same kernels as otfem_step with
+ * stats == NULL (no captured graphs, no timing events; results and errors
+ * through otfem_synchronize), and keeps its per-step scratch -- the tile
+ * partial sums of the element, CG and subspace-start kernels, the
+ * stopped-body and active-tile counters and the active-tile list -- in one
+ * static device array of 25.1 MiB that every GPU the library runs on holds
+ * until the process ends.  Limits and preconditions (OTFEM_INVALID_ARG /
+ * OTFEM_STATE otherwise): at most 32,768 tiles (otfem_layout_info); no
+ * recording; a subspace start or FP64-stored SpMV needs one otfem_step with
+ * the same settings first (it allocates their history and record buffers);
+ * the solver's device is current.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (steps of different solvers in flight on
+ * different streams) race. */
+otfem_status otfem_step_noalloc(otfem_handle h, double dt, int use_stiffness_warping,
+                                const otfem_params *params);
 /* Per-kernel HIP-event timing of every step (element kernel, CG kernels,
  * and the learned-start network when it runs), accumulated until the next
  * otfem_set_timing call.  No reference counterpart (fem has no timing hooks). */

@@ -11,6 +11,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <tuple>
@@ -1185,6 +1186,169 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
   h->pending = true;
   if (!stats) return OTFEM_OK;
   return sync_and_check(h, stats);
+}
+
+}  // extern "C"
+
+// ---- the step without allocation (otfem_step_noalloc) ----
+namespace {
+
+// Scratch of otfem_step_noalloc, for at most kScratchTiles tiles: the tile
+// partial sums of the element kernel (b.b, r.r), of the CG kernels
+// ([2][tile][4]) and of the subspace start's Gram ([tile][kSubPart]), the
+// stopped-body / active-tile counters and the active-tile list.  Offsets in
+// bytes, each 256-aligned.
+constexpr int64_t kScratchTiles = 32768;
+constexpr size_t kScrPartBB = 0;
+constexpr size_t kScrPartRR = kScrPartBB + sizeof(double) * kScratchTiles;
+constexpr size_t kScrPart4 = kScrPartRR + sizeof(double) * kScratchTiles;
+constexpr size_t kScrDone = kScrPart4 + sizeof(double) * 8 * kScratchTiles;
+constexpr size_t kScrAct = kScrDone + 256;
+constexpr size_t kScrSubPart = kScrAct + sizeof(int32_t) * kScratchTiles;
+constexpr size_t kScratchBytes = kScrSubPart + sizeof(double) * kSubPart * kScratchTiles;  // 25.1 MiB
+
+// Each GPU that loads the library holds its own copy until the process ends.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kScratchDevices] = {};
+
+// The device address of `scratch` on the solver's device (looked up once per
+// device, with that device current; host code never uses the symbol itself).
+otfem_status scratch_address(otfem_solver *h, unsigned char **out) {
+  int cur = -1;
+  HIPCHK(h, hipGetDevice(&cur));
+  if (cur != h->device) return fail(h, OTFEM_STATE, "make the solver's device current");
+  if (cur < 0 || cur >= kScratchDevices) return fail(h, OTFEM_NO_DEVICE, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[cur]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(h, e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[cur] = static_cast<unsigned char *>(p);
+  }
+  *out = g_scratch_ptr[cur];
+  return OTFEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+otfem_status otfem_step_noalloc(otfem_handle h, double dt, int use_stiffness_warping,
+                                const otfem_params *params) {
+  if (!h) return OTFEM_INVALID_ARG;
+  if (!h->initialized) return fail(h, OTFEM_STATE, "call otfem_init before otfem_step_noalloc");
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(h, OTFEM_INVALID_ARG, "dt must be positive");
+  otfem_params P;
+  if (params)
+    P = *params;
+  else
+    otfem_default_params(&P, 0);
+  if (P.cg_max_iterations < 0 || P.cg_min_iterations < 0 || !(P.tol >= 0) || !(P.tiny >= 0) ||
+      (P.tol_mode != 0 && P.tol_mode != 1) || (P.rotation_mode != 0 && P.rotation_mode != 1) ||
+      !std::isfinite(P.alpha) || !std::isfinite(P.beta) || P.subspace < 0 || P.subspace > kSubMax - 2 ||
+      (P.warm_start != OTFEM_WARM_PREVIOUS_VELOCITY && P.warm_start != OTFEM_WARM_LEARNED))
+    return fail(h, OTFEM_INVALID_ARG, "bad solver parameters");
+  const int64_t ntiles = (int64_t)h->L.tiles.size();
+  if (ntiles > kScratchTiles)
+    return fail(h, OTFEM_INVALID_ARG,
+                "the mesh has " + std::to_string(ntiles) + " tiles; otfem_step_noalloc takes at most " +
+                    std::to_string(kScratchTiles));
+  // everything but the scratch must exist already
+  const bool learned = P.warm_start == OTFEM_WARM_LEARNED;
+  const bool ws_on = h->model && !h->ws_suspended;
+  if (learned && !ws_on) return fail(h, OTFEM_STATE, "no warm-start model attached");
+  if (ws_on && h->recording) return fail(h, OTFEM_STATE, "recording needs otfem_step");
+  if (h->lc.spmv_mode == SPMV_F64_STORE64 && !h->bt8)
+    return fail(h, OTFEM_STATE, "take one otfem_step in this SpMV mode first (its buffers)");
+  if (P.subspace > 0) {
+    const int dirs = 1 + P.subspace + (learned ? 1 : 0);
+    if (!h->vring || h->ring_slots < P.subspace || !h->prim_begin ||
+        sub_gram_lds_bytes(h->L.max_tile_nodes, h->L.max_tile_owned, dirs) > h->lds_sub)
+      return fail(h, OTFEM_STATE,
+                  "take one otfem_step with these subspace settings first (its history buffers)");
+  }
+  unsigned char *scr = nullptr;
+  otfem_status st = scratch_address(h, &scr);
+  if (st != OTFEM_OK) return st;
+  StepArgs a = make_args(h, dt, use_stiffness_warping, P);
+  a.part_bb = reinterpret_cast<double *>(scr + kScrPartBB);
+  a.part_rr = reinterpret_cast<double *>(scr + kScrPartRR);
+  a.part4 = reinterpret_cast<double *>(scr + kScrPart4);
+  a.done_count = reinterpret_cast<int *>(scr + kScrDone);
+  a.act = reinterpret_cast<int32_t *>(scr + kScrAct);
+  a.sub_part = reinterpret_cast<double *>(scr + kScrSubPart);
+  if (learned && a.sub_hist > 0) {
+    a.sub_net = 1;
+    a.vnet = h->vw;
+  } else if (learned) {
+    a.vw = h->vw;
+  }
+  hipStream_t s = h->stream;
+  if (ws_on) {
+    ModelView mv{};
+    model_view(h->model, &mv);
+    WsArgs w{};
+    w.v = h->v;
+    w.vw = h->vw;
+    w.hist1 = h->hist1;
+    w.hist2 = h->hist2;
+    w.kd = h->kd;
+    w.mass = h->mass;
+    w.fixed = h->fixed;
+    w.free_rank = h->free_rank;
+    w.N = h->L.N;
+    w.cM = 1.0 + P.alpha * dt;
+    w.cK = dt * dt + P.beta * dt;
+    w.predict = learned ? 1 : 0;
+    w.record = 0;
+    w.wf = mv.wf;
+    w.theta = mv.theta;
+    w.bad = h->ws_bad;
+    HIPCHK(h, enqueue_ws_predict(w, s));
+  }
+  // the kernels write every partial sum and counter they read later in the
+  // step (k_element clears the stopped-body count), so nothing is cleared
+  HIPCHK(h, enqueue_element(a, h->lds_elem, s));
+  HIPCHK(h, enqueue_subspace(a, h->lds_sub, h->lds_cg, s));
+  const int cap = std::max(0, P.cg_max_iterations);
+  const int nt = (int)ntiles;
+  int known_active = nt;
+  const int tail_from = h->cg_stop_seen > 0 ? std::max(0, h->cg_stop_seen - 16) : cap;
+  int prev_end = 0;
+  for (int j = 0, it0 = 0; it0 < cap; ++j) {
+    const int len = (it0 % kCgChunk == 0 && it0 + kCgChunk <= tail_from) ? kCgChunk : kCgTail;
+    const int it1 = std::min(cap, it0 + len);
+    int grid = nt;
+    if (a.compact) {
+      grid = 256;
+      while (grid < known_active) grid *= 2;
+      grid = std::min(grid, nt);
+    }
+    HIPCHK(h, enqueue_cg(a, h->lc, h->lds_cg, s, it0, it1, grid));
+    if (it1 < cap) {
+      HIPCHK(h, hipMemcpyAsync(&h->done_host[2 * (j & 1)], a.done_count, 2 * sizeof(int),
+                               hipMemcpyDeviceToHost, s));
+      HIPCHK(h, hipEventRecord(h->chunk_ev[j & 1], s));
+      if (j >= 1) {
+        HIPCHK(h, hipEventSynchronize(h->chunk_ev[(j - 1) & 1]));
+        const volatile int *dh = &h->done_host[2 * ((j - 1) & 1)];
+        if (dh[0] >= (int)h->L.B) {
+          h->cg_stop_seen = prev_end;
+          break;
+        }
+        if (a.compact) known_active = std::min(known_active, std::max((int)dh[1], 1));
+      }
+    }
+    prev_end = it1;
+    it0 = it1;
+  }
+  HIPCHK(h, enqueue_finish(a, h->lc, s));
+  h->rotations_valid = (P.store_rotations != 0) && use_stiffness_warping;
+  h->pending = true;
+  return OTFEM_OK;
 }
 
 otfem_status otfem_synchronize(otfem_handle h, otfem_stats *stats) {

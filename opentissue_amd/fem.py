@@ -143,6 +143,7 @@ SIGNATURES = {
     "otfem_get_timing": (C.c_int, [_H, _dp, _dp, _dp, C.POINTER(C.c_longlong)]),
     "otfem_get_timing_detail": (C.c_int, [_H, _dp, C.POINTER(C.c_longlong)]),
     "otfem_subspace_coefficients": (C.c_int, [_H, _dp]),
+    "otfem_step_noalloc": (C.c_int, [_H, C.c_double, C.c_int, C.POINTER(Params)]),
     "otfem_iteration_totals": (C.c_int, [_H, C.POINTER(C.c_int64), C.c_int]),
     "otfem_unconverged_totals": (C.c_int, [_H, C.POINTER(C.c_int64), C.c_int]),
     "otfem_set_kernel_config": (C.c_int, [_H, C.c_int, C.c_int, C.c_int]),
