@@ -210,6 +210,14 @@ class T4Mesh {
   size_t m_topology_version = 0;
   otfem_params m_params{};
   bool m_params_set = false;
+  // What the device holds, as last written or read back by fem::init /
+  // fem::simulate: fem::simulate uploads only what the caller changed since,
+  // so a caller that leaves a node array alone pays no transfer for it.
+  std::vector<double> m_dev_x, m_dev_v, m_dev_f;
+  std::vector<unsigned char> m_dev_fixed;
+  bool m_dev_valid = false;
+  bool m_plastic = false;         // plasticity adds into m_f_external on the device
+  bool m_read_rotations = true;   // fem::simulate refreshes every m_Re
 
  protected:
   std::vector<Node> m_nodes;
@@ -351,6 +359,7 @@ namespace detail {
 
 template <typename fem_mesh>
 void create_solver(fem_mesh &mesh) {
+  mesh.m_dev_valid = false;  // until finish_init records the new solver's arrays
   const size_t N = mesh.size_nodes(), T = mesh.size_tetrahedra();
   std::vector<double> x(3 * N), x0(3 * N), v(3 * N);
   std::vector<unsigned char> fixed(N);
@@ -382,9 +391,27 @@ void create_solver(fem_mesh &mesh) {
 }
 
 template <typename fem_mesh>
-void finish_init(fem_mesh &mesh) {
+void finish_init(fem_mesh &mesh, bool plastic) {
   otfem_handle h = mesh.m_solver.get();
   const size_t N = mesh.size_nodes(), T = mesh.size_tetrahedra();
+  // the device now holds the mesh's coordinates, velocities and fixed flags
+  // (otfem_create) and zero external forces
+  mesh.m_dev_x.assign(3 * N, 0.0);
+  mesh.m_dev_v.assign(3 * N, 0.0);
+  mesh.m_dev_f.assign(3 * N, 0.0);
+  mesh.m_dev_fixed.assign(N, 0);
+  {
+    size_t j = 0;
+    for (auto n = mesh.node_begin(); n != mesh.node_end(); ++n, ++j) {
+      for (int c = 0; c < 3; ++c) {
+        mesh.m_dev_x[3 * j + c] = n->m_coord(c);
+        mesh.m_dev_v[3 * j + c] = n->m_velocity(c);
+      }
+      mesh.m_dev_fixed[j] = n->m_fixed ? 1 : 0;
+    }
+  }
+  mesh.m_dev_valid = true;
+  mesh.m_plastic = plastic;
   std::vector<double> m(N), V(T);
   ::otfem::detail::check(otfem_get_mass(h, m.data()), h, "otfem_get_mass");
   ::otfem::detail::check(otfem_get_volumes(h, V.data()), h, "otfem_get_volumes");
@@ -414,7 +441,7 @@ inline void init(fem_mesh &mesh, real_type const &young, real_type const &poisso
   otfem_handle h = mesh.m_solver.get();
   ::otfem::detail::check(otfem_init(h, young, poisson, density, c_yield, c_creep, c_max), h,
                          "fem::init");
-  detail::finish_init(mesh);
+  detail::finish_init(mesh, c_creep > 0 && c_max > 0);
 }
 
 // The user-written init of fem_init.h:32-35: materials already set per tet
@@ -432,7 +459,7 @@ inline void init_materials(fem_mesh &mesh, real_type const &c_yield, real_type c
   otfem_handle h = mesh.m_solver.get();
   ::otfem::detail::check(otfem_init_materials(h, E.data(), nu.data(), rho.data(), c_yield,
                                               c_creep, c_max), h, "fem::init_materials");
-  detail::finish_init(mesh);
+  detail::finish_init(mesh, c_creep > 0 && c_max > 0);
 }
 
 // Solver settings for subsequent fem::simulate calls on this mesh (the
@@ -443,14 +470,26 @@ inline void set_params(fem_mesh &mesh, const otfem_params &p) {
   mesh.m_params_set = true;
 }
 
-// fem_simulate.h:49-113
+// Whether fem::simulate refreshes every tetrahedron's m_Re (default, as the
+// reference's update_orientation does).  A caller that never reads m_Re can
+// turn it off: the device then stores no rotations and none cross the bus.
+template <typename fem_mesh>
+inline void set_rotation_readback(fem_mesh &mesh, bool on) {
+  mesh.m_read_rotations = on;
+}
+
+// fem_simulate.h:49-113.  The caller's node arrays are compared with what the
+// device already holds and only changed ones are uploaded (a velocity upload
+// restarts the learned and subspace starts' histories, a position or force
+// upload does not); positions and velocities come back every step, external
+// forces only when plasticity added into them, rotations unless turned off.
 template <typename fem_mesh, typename real_type>
 inline void simulate(fem_mesh &mesh, real_type const &time_step, bool use_stiffness_warping) {
   otfem_handle h = mesh.m_solver.get();
-  if (!h || mesh.m_solver_version != mesh.m_topology_version)
+  if (!h || mesh.m_solver_version != mesh.m_topology_version || !mesh.m_dev_valid)
     throw std::logic_error("fem::simulate(): fem::init must be called after the mesh is built");
   const size_t N = mesh.size_nodes(), T = mesh.size_tetrahedra();
-  std::vector<double> x(3 * N), v(3 * N), f(3 * N), Re(9 * T);
+  std::vector<double> x(3 * N), v(3 * N), f(3 * N);
   std::vector<unsigned char> fixed(N);
   size_t i = 0;
   for (auto n = mesh.node_begin(); n != mesh.node_end(); ++n, ++i) {
@@ -461,32 +500,49 @@ inline void simulate(fem_mesh &mesh, real_type const &time_step, bool use_stiffn
     }
     fixed[i] = n->m_fixed ? 1 : 0;
   }
-  ::otfem::detail::check(otfem_set_fixed(h, fixed.data()), h, "otfem_set_fixed");
-  ::otfem::detail::check(otfem_set_state(h, x.data(), v.data()), h, "otfem_set_state");
-  ::otfem::detail::check(otfem_set_external_forces(h, f.data()), h, "otfem_set_external_forces");
+  if (fixed != mesh.m_dev_fixed) {
+    ::otfem::detail::check(otfem_set_fixed(h, fixed.data()), h, "otfem_set_fixed");
+    mesh.m_dev_fixed = fixed;
+  }
+  const bool x_new = x != mesh.m_dev_x, v_new = v != mesh.m_dev_v;
+  if (x_new || v_new)
+    ::otfem::detail::check(otfem_set_state(h, x_new ? x.data() : nullptr, v_new ? v.data() : nullptr),
+                           h, "otfem_set_state");
+  if (f != mesh.m_dev_f) {
+    ::otfem::detail::check(otfem_set_external_forces(h, f.data()), h, "otfem_set_external_forces");
+    mesh.m_dev_f = f;
+  }
   otfem_params P;
   if (mesh.m_params_set)
     P = mesh.m_params;
   else
     otfem_default_params(&P, 0);
-  P.store_rotations = 1;
+  P.store_rotations = mesh.m_read_rotations ? 1 : 0;
   otfem_stats st;
   ::otfem::detail::check(otfem_step(h, (double)time_step, use_stiffness_warping ? 1 : 0, &P, &st),
                          h, "fem::simulate");
   ::otfem::detail::check(otfem_get_state(h, x.data(), v.data()), h, "otfem_get_state");
-  ::otfem::detail::check(otfem_get_external_forces(h, f.data()), h, "otfem_get_external_forces");
-  ::otfem::detail::check(otfem_get_rotations(h, Re.data()), h, "otfem_get_rotations");
+  if (mesh.m_plastic) {
+    ::otfem::detail::check(otfem_get_external_forces(h, f.data()), h, "otfem_get_external_forces");
+    mesh.m_dev_f = f;
+  }
   i = 0;
   for (auto n = mesh.node_begin(); n != mesh.node_end(); ++n, ++i)
     for (int c = 0; c < 3; ++c) {
       n->m_coord(c) = x[3 * i + c];
       n->m_velocity(c) = v[3 * i + c];
-      n->m_f_external(c) = f[3 * i + c];
+      if (mesh.m_plastic) n->m_f_external(c) = f[3 * i + c];
     }
-  size_t t = 0;
-  for (auto T4 = mesh.tetrahedron_begin(); T4 != mesh.tetrahedron_end(); ++T4, ++t)
-    for (int r = 0; r < 3; ++r)
-      for (int c = 0; c < 3; ++c) T4->m_Re(r, c) = Re[9 * t + 3 * r + c];
+  mesh.m_dev_x.swap(x);
+  mesh.m_dev_v.swap(v);
+  if (mesh.m_read_rotations) {
+    std::vector<double> Re(9 * T);
+    ::otfem::detail::check(otfem_get_rotations(h, Re.data()), h, "otfem_get_rotations");
+    size_t t = 0;
+    for (auto T4 = mesh.tetrahedron_begin(); T4 != mesh.tetrahedron_end(); ++T4, ++t)
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) T4->m_Re(r, c) = Re[9 * t + 3 * r + c];
+  }
 }
 
 // fem_update_original_coord.h:28-33 (host-side; fem::init must follow, as in

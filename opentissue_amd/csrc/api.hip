@@ -428,49 +428,65 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
   const int64_t T = L.T, S = (int64_t)L.slot_tet.size(), N = L.N;
   if (!(c_yield >= 0) || !(c_creep >= 0) || !(c_max >= 0))
     return fail(h, OTFEM_INVALID_ARG, "plastic parameters must be non-negative");
-  h->young.resize(T);
-  h->poisson.resize(T);
-  h->density.resize(T);
+  // everything is checked before the solver changes: a rejected init leaves
+  // the previous materials, element data and masses in place
+  std::vector<double> young(T), poisson(T), density(T);
   for (int64_t t = 0; t < T; ++t) {
     double e = uniform ? E[0] : E[t], n = uniform ? nu[0] : nu[t], r = uniform ? rho[0] : rho[t];
     // fem_uniform_young.h:32, fem_uniform_poisson.h:32-33, density > 0; nu = 0.5
     // is rejected as well since it makes D infinite.
     if (!(e >= 0) || !(n >= 0) || !(n < 0.5) || !(r > 0) || !std::isfinite(e) || !std::isfinite(r))
       return fail(h, OTFEM_INVALID_ARG, "material out of range (E>=0, 0<=nu<0.5, rho>0)");
-    h->young[t] = e;
-    h->poisson[t] = n;
-    h->density[t] = r;
+    young[t] = e;
+    poisson[t] = n;
+    density[t] = r;
   }
-  otfem_status st = ensure_device(h);
-  if (st != OTFEM_OK) return st;
   std::vector<ElemData> ed(T);
-  h->vol_host.resize(T);
+  std::vector<double> vol(T);
   int64_t inverted = 0;
   const double *x0 = h->x0_host.data();
   for (int64_t t = 0; t < T; ++t) {
     const int32_t *tt = &h->tets[4 * t];
-    ed[t] = element_data(x0 + 3 * tt[0], x0 + 3 * tt[1], x0 + 3 * tt[2], x0 + 3 * tt[3],
-                         h->young[t], h->poisson[t]);
-    h->vol_host[t] = ed[t].V;
+    ed[t] = element_data(x0 + 3 * tt[0], x0 + 3 * tt[1], x0 + 3 * tt[2], x0 + 3 * tt[3], young[t],
+                         poisson[t]);
+    vol[t] = ed[t].V;
     if (!(ed[t].V > 0)) ++inverted;
   }
   if (inverted)
     return fail(h, OTFEM_INVERTED_TET,
                 std::to_string(inverted) + " tetrahedra have non-positive rest volume");
   // fem_compute_mass.h:154-171 in caller order
-  h->mass_host.assign(N, 0.0);
-  for (int64_t i = 0; i < N; ++i)
-    h->mass_host[i] = h->fixed_host[i] ? std::numeric_limits<double>::max() : 0.0;
+  std::vector<double> mass(N, 0.0);
+  for (int64_t i = 0; i < N; ++i) mass[i] = h->fixed_host[i] ? std::numeric_limits<double>::max() : 0.0;
   for (int64_t t = 0; t < T; ++t) {
-    double amount = h->density[t] * ed[t].V * .25;
-    for (int k = 0; k < 4; ++k) h->mass_host[h->tets[4 * t + k]] += amount;
+    double amount = density[t] * ed[t].V * .25;
+    for (int k = 0; k < 4; ++k) mass[h->tets[4 * t + k]] += amount;
   }
   // A free node outside every tetrahedron has zero mass and an empty row of A;
   // the reference would divide by zero in CG.  Reject it (fix or drop it).
   for (int64_t i = 0; i < N; ++i)
-    if (!h->fixed_host[i] && !(h->mass_host[i] > 0.0))
+    if (!h->fixed_host[i] && !(mass[i] > 0.0))
       return fail(h, OTFEM_INVALID_ARG,
                   "free node " + std::to_string(i) + " belongs to no tetrahedron (zero mass)");
+  otfem_status st = ensure_device(h);
+  if (st != OTFEM_OK) return st;
+  const size_t lds_elem = element_lds_bytes(L.max_tile_nodes, L.max_tile_owned,
+                                            (c_creep > 0.0 && c_max > 0.0) ? 1 : 0);
+  const size_t lds_cg = cg_lds_bytes(L.max_tile_nodes, L.max_tile_owned);
+  {
+    hipDeviceProp_t prop;
+    HIPCHK(h, hipGetDeviceProperties(&prop, h->device));
+    // the kernels' static LDS (block reductions) is under 1 KiB
+    constexpr size_t kStaticLds = 1024;
+    if (lds_elem + kStaticLds > (size_t)prop.sharedMemPerBlock ||
+        lds_cg + kStaticLds > (size_t)prop.sharedMemPerBlock)
+      return fail(h, OTFEM_INVALID_ARG, "tile too large for LDS; lower target_tile_nodes");
+  }
+  h->young.swap(young);
+  h->poisson.swap(poisson);
+  h->density.swap(density);
+  h->vol_host.swap(vol);
+  h->mass_host.swap(mass);
   // Diagonal of the rest stiffness K_ii (R = I): per tet and corner k with
   // gradient b, (Ke_kk)_dd = D0 b_d^2 + D2 (|b|^2 - b_d^2) (fem_compute_ke.h:78-99,
   // with D = V (d16, d17, d18)); feature of the learned warm start.
@@ -549,15 +565,8 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
   HIPCHK(h, hipMemsetAsync(h->plastic, 0, 6 * T * sizeof(double), s));
   HIPCHK(h, hipStreamSynchronize(s));
 
-  h->lds_elem = element_lds_bytes(L.max_tile_nodes, L.max_tile_owned, h->plastic_on ? 1 : 0);
-  h->lds_cg = cg_lds_bytes(L.max_tile_nodes, L.max_tile_owned);
-  hipDeviceProp_t prop;
-  HIPCHK(h, hipGetDeviceProperties(&prop, h->device));
-  // the kernels' static LDS (block reductions) is under 1 KiB
-  constexpr size_t kStaticLds = 1024;
-  if (h->lds_elem + kStaticLds > (size_t)prop.sharedMemPerBlock ||
-      h->lds_cg + kStaticLds > (size_t)prop.sharedMemPerBlock)
-    return fail(h, OTFEM_INVALID_ARG, "tile too large for LDS; lower target_tile_nodes");
+  h->lds_elem = lds_elem;
+  h->lds_cg = lds_cg;
   // material data changed: captured graphs are rebuilt on the next step
   drop_graphs(h);
   h->initialized = true;
@@ -1483,37 +1492,62 @@ otfem_status otfem_set_deterministic(otfem_handle h, int on) {
 }
 
 // Block-size sweep of otfem_autotune; *best_ms is the element + CG device
-// time of the `steps` timed steps at the chosen block size.
+// time of the `steps` timed steps at the chosen block size.  The trial steps
+// leave the solver as it was: node state, plastic strain, the per-body
+// iteration and unconverged counters, the last step's body records, the
+// rotation flag, the chunking hint and the timing totals are restored on
+// every exit; the trials store no rotations and run the plain start (so the
+// learned and subspace histories are not touched).
 static otfem_status tune_blocks(otfem_handle h, double dt, int use_stiffness_warping,
                                 const otfem_params *params, int steps, int *best_block,
                                 double *best_ms) {
   if (!h) return OTFEM_INVALID_ARG;
   if (!h->initialized) return fail(h, OTFEM_STATE, "call otfem_init before otfem_autotune");
   if (steps <= 0) steps = 2;
-  const int64_t N = h->L.N, T = h->L.T;
-  // the trial steps must leave the caller's state untouched
+  const int64_t N = h->L.N, T = h->L.T, B = h->L.B;
   double *sx = nullptr, *sv = nullptr, *sf = nullptr, *spl = nullptr;
+  unsigned long long *sit = nullptr, *sun = nullptr;
   hipError_t ae = dalloc(&sx, 3 * N);
   if (ae == hipSuccess) ae = dalloc(&sv, 3 * N);
   if (ae == hipSuccess) ae = dalloc(&sf, 3 * N);
   if (ae == hipSuccess) ae = dalloc(&spl, 6 * T);
+  if (ae == hipSuccess) ae = dalloc(&sit, B);
+  if (ae == hipSuccess) ae = dalloc(&sun, B);
+  auto release = [&]() {
+    dfree(sx); dfree(sv); dfree(sf); dfree(spl); dfree(sit); dfree(sun);
+  };
   if (ae != hipSuccess) {
-    dfree(sx); dfree(sv); dfree(sf); dfree(spl);
+    release();
     return hip_fail(h, ae, "autotune: state copies");
   }
   hipStream_t s = h->stream;
-  auto save = [&](bool restore) -> hipError_t {
-    auto copy = [&](double *live, double *kept, int64_t count) {
-      return hipMemcpyAsync(restore ? live : kept, restore ? kept : live, count * sizeof(double),
+  auto copy_state = [&](bool restore) -> hipError_t {
+    auto copy = [&](auto *live, auto *kept, int64_t count) {
+      return hipMemcpyAsync(restore ? live : kept, restore ? kept : live, count * sizeof(*live),
                             hipMemcpyDeviceToDevice, s);
     };
     hipError_t e;
     if ((e = copy(h->x, sx, 3 * N)) || (e = copy(h->v, sv, 3 * N)) || (e = copy(h->fext, sf, 3 * N)) ||
-        (e = copy(h->plastic, spl, 6 * T)))
+        (e = copy(h->plastic, spl, 6 * T)) || (e = copy(h->iter_acc, sit, B)) ||
+        (e = copy(h->unconv_acc, sun, B)))
       return e;
     return hipStreamSynchronize(s);
   };
-  // trials use the plain CG start and leave the learned-start history alone
+  // steps enqueued before: finish them so their records and timings are kept
+  if (h->pending) {
+    otfem_status ps = sync_and_check(h, nullptr);
+    if (ps != OTFEM_OK) {
+      release();
+      return ps;
+    }
+  }
+  // host-side records of the solver
+  const std::vector<BodyState> kept_body = h->last_body;
+  const bool kept_rot = h->rotations_valid;
+  const int kept_stop = h->cg_stop_seen;
+  const double kept_ms[4] = {h->acc_elem_ms, h->acc_cg_ms, h->acc_ws_ms, h->acc_sub_ms};
+  const long long kept_steps = h->timed_steps;
+  const bool timing_was = h->timing;
   otfem_params P;
   if (params)
     P = *params;
@@ -1521,37 +1555,50 @@ static otfem_status tune_blocks(otfem_handle h, double dt, int use_stiffness_war
     otfem_default_params(&P, 0);
   P.warm_start = OTFEM_WARM_PREVIOUS_VELOCITY;
   P.subspace = 0;
-  params = &P;
+  P.store_rotations = 0;
   h->ws_suspended = true;
   otfem_status ret = OTFEM_OK;
   const int blocks[3] = {256, 512, 1024};
   double best = 1e300;
   int best_b = h->lc.cg_block;
-  const bool timing_was = h->timing;
-  if (save(false) != hipSuccess) ret = fail(h, OTFEM_HIP_ERROR, "autotune: state copy failed");
+  if (copy_state(false) != hipSuccess) ret = fail(h, OTFEM_HIP_ERROR, "autotune: state copy failed");
   for (int bi = 0; bi < 3 && ret == OTFEM_OK; ++bi) {
     h->lc.cg_block = blocks[bi];
     otfem_stats st;
-    ret = otfem_step(h, dt, use_stiffness_warping, params, &st);  // warm-up, builds the graph
+    ret = otfem_step(h, dt, use_stiffness_warping, &P, &st);  // warm-up, builds the graphs
+    if (ret == OTFEM_OK && copy_state(true) != hipSuccess)
+      ret = fail(h, OTFEM_HIP_ERROR, "autotune: restore failed");
     if (ret != OTFEM_OK) break;
-    if (save(true) != hipSuccess) { ret = fail(h, OTFEM_HIP_ERROR, "autotune: restore failed"); break; }
     h->timing = true;
     const double e0 = h->acc_elem_ms + h->acc_cg_ms;
-    for (int k = 0; k < steps && ret == OTFEM_OK; ++k)
-      ret = otfem_step(h, dt, use_stiffness_warping, params, nullptr);
+    for (int k = 0; k < steps && ret == OTFEM_OK; ++k) ret = otfem_step(h, dt, use_stiffness_warping, &P, nullptr);
     if (ret == OTFEM_OK) ret = sync_and_check(h, nullptr);
     const double ms = h->acc_elem_ms + h->acc_cg_ms - e0;
     h->timing = timing_was;
-    if (save(true) != hipSuccess) { ret = fail(h, OTFEM_HIP_ERROR, "autotune: restore failed"); break; }
+    if (ret == OTFEM_OK && copy_state(true) != hipSuccess)
+      ret = fail(h, OTFEM_HIP_ERROR, "autotune: restore failed");
     if (ret == OTFEM_OK && ms < best) {
       best = ms;
       best_b = blocks[bi];
     }
   }
+  // restore on every path (a failed trial may have left partial state)
+  if (ret != OTFEM_OK) {
+    (void)hipStreamSynchronize(s);
+    (void)copy_state(true);
+  }
   h->timing = timing_was;
   h->ws_suspended = false;
   h->lc.cg_block = best_b;
-  dfree(sx); dfree(sv); dfree(sf); dfree(spl);
+  h->last_body = kept_body;
+  h->rotations_valid = kept_rot;
+  h->cg_stop_seen = kept_stop;
+  h->acc_elem_ms = kept_ms[0];
+  h->acc_cg_ms = kept_ms[1];
+  h->acc_ws_ms = kept_ms[2];
+  h->acc_sub_ms = kept_ms[3];
+  h->timed_steps = kept_steps;
+  release();
   if (best_block) *best_block = best_b;
   if (best_ms) *best_ms = best;
   return ret;
@@ -1859,10 +1906,28 @@ static otfem_status otfem_save_state_body(otfem_handle h, const char *path) {
   std::vector<unsigned long long> it(B), un(B);
   HIPCHK(h, hipMemcpy(it.data(), h->iter_acc, B * sizeof(unsigned long long), hipMemcpyDeviceToHost));
   HIPCHK(h, hipMemcpy(un.data(), h->unconv_acc, B * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  // the subspace start's history: ring of velocity differences, v^n-1 / v^n
+  const bool sub = h->vring && h->sub_state && h->vbuf;
+  int32_t sub_head[3] = {0, 0, 0};
+  std::vector<std::vector<float>> ring;
+  std::vector<double> vb0(3 * N), vb1(3 * N);
+  if (sub) {
+    HIPCHK(h, hipMemcpy(sub_head, h->sub_state, sizeof(sub_head), hipMemcpyDeviceToHost));
+    ring.resize(h->ring_slots);
+    for (int k = 0; k < h->ring_slots; ++k) {
+      HIPCHK(h, hipMemcpy(raw.data(), h->vring + (int64_t)k * N, N * sizeof(float4), hipMemcpyDeviceToHost));
+      hist_to_caller(h->L, raw, ring[k]);
+    }
+    for (auto [dst, half] : {std::pair<double *, int>{vb0.data(), 0}, {vb1.data(), 1}}) {
+      HIPCHK(h, hipMemcpy(tmp.data(), h->vbuf + (int64_t)half * 3 * N, 3 * N * sizeof(double),
+                          hipMemcpyDeviceToHost));
+      to_caller3(h->L, tmp, dst);
+    }
+  }
   std::ofstream o(path, std::ios::binary);
   if (!o) return fail(h, OTFEM_INVALID_ARG, std::string("cannot write ") + path);
   auto put = [&](const void *p, size_t n) { o.write(static_cast<const char *>(p), (std::streamsize)n); };
-  const uint32_t version = 1, flags = hist ? 1u : 0u;
+  const uint32_t version = 1, flags = (hist ? 1u : 0u) | (sub ? 2u : 0u);
   const int64_t dims[3] = {N, T, B};
   const uint64_t hash = connectivity_hash(h);
   put("OTFEMSV1", 8);
@@ -1886,6 +1951,14 @@ static otfem_status otfem_save_state_body(otfem_handle h, const char *path) {
   }
   put(it.data(), B * 8);
   put(un.data(), B * 8);
+  if (sub) {
+    const int32_t slots = h->ring_slots;
+    put(&slots, 4);
+    put(sub_head, sizeof(sub_head));
+    for (const auto &r : ring) put(r.data(), 4 * N * 4);
+    put(vb0.data(), 3 * N * 8);
+    put(vb1.data(), 3 * N * 8);
+  }
   if (!o) return fail(h, OTFEM_INVALID_ARG, std::string("write failed: ") + path);
   return OTFEM_OK;
 }
@@ -1935,21 +2008,52 @@ static otfem_status otfem_load_state_body(otfem_handle h, const char *path) {
   }
   get(it.data(), B * 8);
   get(un.data(), B * 8);
+  int32_t slots = 0, sub_head[3] = {0, 0, 0};
+  std::vector<std::vector<float>> ring;
+  std::vector<double> vb0, vb1;
+  if (flags & 2u) {
+    get(&slots, 4);
+    get(sub_head, sizeof(sub_head));
+    if (!in || slots < 1 || slots > kSubMax - 2 || sub_head[0] < 0 || sub_head[0] >= slots ||
+        sub_head[1] < 0 || sub_head[1] > slots + 1 || (sub_head[2] & ~1))
+      return fail(h, OTFEM_INVALID_ARG, std::string("bad subspace history in ") + path);
+    ring.assign(slots, std::vector<float>(4 * N));
+    for (auto &r : ring) get(r.data(), 4 * N * 4);
+    vb0.resize(3 * N);
+    vb1.resize(3 * N);
+    get(vb0.data(), 3 * N * 8);
+    get(vb1.data(), 3 * N * 8);
+  }
   if (!in) return fail(h, OTFEM_INVALID_ARG, std::string("state file is truncated: ") + path);
   otfem_status st = ensure_device(h);
   if (st) return st;
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  // rest shape and fixed flags first: the element data and masses follow them
-  h->x0_host = x0;
-  for (int64_t i = 0; i < N; ++i) h->fixed_host[i] = fixed[i] ? 1 : 0;
+  // rest shape and fixed flags first: the element data and masses follow
+  // them.  init_common checks everything before it changes the solver; if it
+  // rejects the file, the previous rest shape and flags are put back.
   std::vector<double> tmp;
   std::vector<uint8_t> fx(N);
-  for (int64_t g = 0; g < N; ++g) fx[g] = h->fixed_host[h->L.node_of_internal[g]];
-  to_internal3(h->L, x0.data(), tmp);
-  HIPCHK(h, hipMemcpy(h->x0, tmp.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemcpy(h->fixed, fx.data(), N, hipMemcpyHostToDevice));
+  auto put_shape = [&]() -> hipError_t {
+    for (int64_t g = 0; g < N; ++g) fx[g] = h->fixed_host[h->L.node_of_internal[g]];
+    to_internal3(h->L, h->x0_host.data(), tmp);
+    hipError_t e = hipMemcpy(h->x0, tmp.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->fixed, fx.data(), N, hipMemcpyHostToDevice);
+    return e;
+  };
+  std::vector<double> old_x0 = h->x0_host;
+  std::vector<uint8_t> old_fixed = h->fixed_host;
+  h->x0_host = x0;
+  for (int64_t i = 0; i < N; ++i) h->fixed_host[i] = fixed[i] ? 1 : 0;
+  HIPCHK(h, put_shape());
   st = init_common(h, E.data(), nu.data(), rho.data(), false, pc[0], pc[1], pc[2]);
-  if (st) return st;
+  if (st) {
+    const std::string why = h->err;
+    h->x0_host.swap(old_x0);
+    h->fixed_host.swap(old_fixed);
+    HIPCHK(h, put_shape());
+    h->err = why;
+    return st;
+  }
   if (h->free_rank) {
     st = ws_upload_free(h);
     if (st) return st;
@@ -1969,6 +2073,28 @@ static otfem_status otfem_load_state_body(otfem_handle h, const char *path) {
   }
   HIPCHK(h, hipMemcpy(h->iter_acc, it.data(), B * sizeof(unsigned long long), hipMemcpyHostToDevice));
   HIPCHK(h, hipMemcpy(h->unconv_acc, un.data(), B * sizeof(unsigned long long), hipMemcpyHostToDevice));
+  if (flags & 2u) {
+    // the subspace start continues from the saved history (init_common above
+    // cleared the solver's own); the ring keeps the file's slot count
+    if (h->vring && h->ring_slots != slots) {
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      drop_graphs(h);
+      dfree(h->vring);
+      h->ring_slots = 0;
+    }
+    st = sub_ensure(h, slots, 2);
+    if (st) return st;
+    std::vector<float> raw;
+    for (int k = 0; k < slots; ++k) {
+      hist_to_internal(h->L, ring[k], raw);
+      HIPCHK(h, hipMemcpy(h->vring + (int64_t)k * N, raw.data(), N * sizeof(float4), hipMemcpyHostToDevice));
+    }
+    to_internal3(h->L, vb0.data(), tmp);
+    HIPCHK(h, hipMemcpy(h->vbuf, tmp.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice));
+    to_internal3(h->L, vb1.data(), tmp);
+    HIPCHK(h, hipMemcpy(h->vbuf + 3 * N, tmp.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->sub_state, sub_head, sizeof(sub_head), hipMemcpyHostToDevice));
+  }
   h->rotations_valid = false;
   drop_graphs(h);
   return OTFEM_OK;

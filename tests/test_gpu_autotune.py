@@ -44,19 +44,34 @@ def _run(c, t, fixed, f, E, P, steps, tile=0, block=0):
 
 
 def test_block_autotune_restores_state():
+    """Node state, rotations, per-body iteration / unconverged totals, the
+    last step's body records and the timing totals are the same before and
+    after the trials (which run steps of their own)."""
     c, t, fixed, f, E = _cube(12)
-    P = fem.config_params()
+    P = fem.config_params(store_rotations=1)
     s = fem.Solver(c, t, fixed)
     s.init_materials(E, np.full(len(t), 0.3), np.full(len(t), 1000.0))
     s.set_external_forces(f)
+    s.set_timing(True)
     s.simulate(0.005, True, P)
+    s.step_async(0.005, True, P)  # pending when the autotuner starts
+    s.synchronize()
     x0, v0 = s.state()
+    R0 = s.rotations()
+    it0, un0 = s.iteration_totals(), s.unconverged_totals()
+    bs0 = s.body_stats()
+    tm0 = s.timing_detail()
     block = s.autotune(0.005, True, P, steps=2)
     print("block", block, flush=True)
     assert block in (256, 512, 1024)
     x1, v1 = s.state()
     assert np.array_equal(x0, x1) and np.array_equal(v0, v1)
     assert np.array_equal(s.external_forces(), f)
+    assert np.array_equal(s.rotations(), R0) and not np.allclose(R0, np.eye(3))
+    assert np.array_equal(s.iteration_totals(), it0) and np.array_equal(s.unconverged_totals(), un0)
+    bs1 = s.body_stats()
+    assert all(np.array_equal(a, b) for a, b in zip(bs0, bs1))
+    assert s.timing_detail() == tm0
     s.close()
 
 

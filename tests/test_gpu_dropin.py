@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 EXE = os.path.join(ROOT, "tests", "cpp", "build", "dropin_c1")
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_cpp_dropin_matches_oracle_per_step(tmp_path, mode):
     import oracle
     from opentissue_amd import mesh
@@ -40,12 +40,20 @@ def test_cpp_dropin_matches_oracle_per_step(tmp_path, mode):
     m = o.mass()
     f = mesh.gravity_forces(np.where(fixed == 1, 0.0, m), fixed) * (50.0 if mode == 2 else 1.0)
     P = oracle.reference_params() if mode == 0 else oracle.config_params()
+    if mode == 3:
+        P = oracle.config_params(tol=1e-10, cap=5000)
     np.testing.assert_array_equal(states[0, 0], c)
     errs = []
     for k in range(steps):
         # the caller resets m_f_external before every step
         o.set_external_forces(f)
-        o.set_state(states[k, 0], states[k, 1])
+        xs, vs = states[k, 0].copy(), states[k, 1].copy()
+        if mode == 3:  # the caller's edits before step k (x only, then v only)
+            if k % 2 == 1:
+                xs[fixed == 0, 0] += 1e-4 * np.sin(np.arange(N))[fixed == 0]
+            if k == 2:
+                vs *= 0.5
+        o.set_state(xs, vs)
         o.simulate(0.01, True, P)
         xo, _ = o.state()
         errs.append(rel_disp_error(states[k + 1, 0], xo, c))
@@ -62,3 +70,20 @@ def test_cpp_dropin_matches_oracle_per_step(tmp_path, mode):
     Kff = Kd[np.ix_(fr, fr)]
     blocks = (np.abs(Kff.reshape(fr.sum() // 3, 3, fr.sum() // 3, 3)).sum((1, 3)) > 0).sum()
     assert nk == 9 * blocks + 3 * int(fixed.sum())
+
+
+def test_cpp_dropin_timing_mode():
+    """dropin_c1 --time: fem::simulate with and without the rotation
+    read-back, then otfem_step alone on the same solver."""
+    import json
+    assert os.path.exists(EXE), "build() compiles tests/cpp/dropin_c1.cpp"
+    for rot in (1, 0):
+        r = subprocess.run([EXE, "--time", "6", "6", "6", "4", "2", str(rot)],
+                           capture_output=True, text=True, timeout=120)
+        print(r.stdout, r.stderr)
+        assert r.returncode == 0
+        d = json.loads(r.stdout.strip().splitlines()[-1])
+        assert d["tets"] == 5 * 6 ** 3 and d["rotations"] == rot
+        for k in ("dropin_ms_per_step", "direct_ms_per_step", "dropin_kernel_ms_per_step",
+                  "direct_kernel_ms_per_step", "dropin_cg_iterations_per_step"):
+            assert d[k] > 0, k
