@@ -215,6 +215,9 @@ class T4Mesh {
   // so a caller that leaves a node array alone pays no transfer for it.
   std::vector<double> m_dev_x, m_dev_v, m_dev_f;
   std::vector<unsigned char> m_dev_fixed;
+  // fem::simulate's gather and read-back buffers, kept between steps
+  std::vector<double> m_host_x, m_host_v, m_host_f, m_host_Re;
+  std::vector<unsigned char> m_host_fixed;
   bool m_dev_valid = false;
   bool m_plastic = false;         // plasticity adds into m_f_external on the device
   bool m_read_rotations = true;   // fem::simulate refreshes every m_Re
@@ -489,8 +492,12 @@ inline void simulate(fem_mesh &mesh, real_type const &time_step, bool use_stiffn
   if (!h || mesh.m_solver_version != mesh.m_topology_version || !mesh.m_dev_valid)
     throw std::logic_error("fem::simulate(): fem::init must be called after the mesh is built");
   const size_t N = mesh.size_nodes(), T = mesh.size_tetrahedra();
-  std::vector<double> x(3 * N), v(3 * N), f(3 * N);
-  std::vector<unsigned char> fixed(N);
+  std::vector<double> &x = mesh.m_host_x, &v = mesh.m_host_v, &f = mesh.m_host_f;
+  std::vector<unsigned char> &fixed = mesh.m_host_fixed;
+  x.resize(3 * N);
+  v.resize(3 * N);
+  f.resize(3 * N);
+  fixed.resize(N);
   size_t i = 0;
   for (auto n = mesh.node_begin(); n != mesh.node_end(); ++n, ++i) {
     for (int c = 0; c < 3; ++c) {
@@ -536,7 +543,8 @@ inline void simulate(fem_mesh &mesh, real_type const &time_step, bool use_stiffn
   mesh.m_dev_x.swap(x);
   mesh.m_dev_v.swap(v);
   if (mesh.m_read_rotations) {
-    std::vector<double> Re(9 * T);
+    std::vector<double> &Re = mesh.m_host_Re;
+    Re.resize(9 * T);
     ::otfem::detail::check(otfem_get_rotations(h, Re.data()), h, "otfem_get_rotations");
     size_t t = 0;
     for (auto T4 = mesh.tetrahedron_begin(); T4 != mesh.tetrahedron_end(); ++T4, ++t)

@@ -41,6 +41,14 @@ struct otfem_solver {
   std::vector<double> young, poisson, density;  // per tet, from the last init
   double plastic_c[3] = {0.0, 0.0, 0.0};        // yield, creep, max of the last init
   std::vector<double> mass_host, vol_host;
+  // staging of node arrays in internal order for the set/get calls, kept
+  // between calls so that per-step transfers reuse resident pages
+  std::vector<double> stage;
+  // two pinned halves of pin_half doubles for device-to-host reads of node
+  // and tet arrays (pinned_read)
+  double *pin_stage = nullptr;
+  size_t pin_half = 0;
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};
   bool device_ready = false;
   bool initialized = false;
   bool plastic_on = false;
@@ -201,6 +209,13 @@ void free_device(otfem_solver *h) {
   drop_graphs(h);
   if (h->done_host) (void)hipHostFree(h->done_host);
   h->done_host = nullptr;
+  if (h->pin_stage) (void)hipHostFree(h->pin_stage);
+  h->pin_stage = nullptr;
+  h->pin_half = 0;
+  for (hipEvent_t &e : h->pin_ev) {
+    if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
   for (hipEvent_t &e : h->chunk_ev) {
     if (e) (void)hipEventDestroy(e);
     e = nullptr;
@@ -237,6 +252,56 @@ void to_caller3(const Layout &L, const std::vector<double> &src, double *dst) {
     dst[3 * u + 1] = src[3 * g + 1];
     dst[3 * u + 2] = src[3 * g + 2];
   }
+}
+
+// Reads n doubles from device memory through the pinned halves: chunk k+1
+// copies into one half while `sink(offset, chunk, count)` consumes chunk k
+// from the other.  Halves hold at most kPinHalf doubles, a multiple of 9, so
+// chunks hold whole nodes (3 doubles) and whole rotations (9).
+constexpr size_t kPinHalf = size_t(9) << 18;  // 18 MiB
+
+template <typename Sink>
+otfem_status pinned_read(otfem_solver *h, const double *src, size_t n, Sink &&sink) {
+  if (n == 0) return OTFEM_OK;
+  const size_t want = std::min(kPinHalf, (n + 8) / 9 * 9);
+  if (h->pin_half < want) {
+    if (h->pin_stage) HIPCHK(h, hipHostFree(h->pin_stage));
+    h->pin_stage = nullptr;
+    h->pin_half = 0;
+    HIPCHK(h, hipHostMalloc(reinterpret_cast<void **>(&h->pin_stage), 2 * want * sizeof(double),
+                            hipHostMallocDefault));
+    h->pin_half = want;
+  }
+  for (hipEvent_t &e : h->pin_ev)
+    if (!e) HIPCHK(h, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  const size_t half = h->pin_half, chunks = (n + half - 1) / half;
+  auto issue = [&](size_t k) -> hipError_t {
+    const size_t off = k * half, len = std::min(half, n - off);
+    hipError_t e = hipMemcpyAsync(h->pin_stage + (k & 1) * half, src + off, len * sizeof(double),
+                                  hipMemcpyDeviceToHost, h->stream);
+    return e != hipSuccess ? e : hipEventRecord(h->pin_ev[k & 1], h->stream);
+  };
+  HIPCHK(h, issue(0));
+  for (size_t k = 0; k < chunks; ++k) {
+    if (k + 1 < chunks) HIPCHK(h, issue(k + 1));  // its half was consumed at k - 1
+    HIPCHK(h, hipEventSynchronize(h->pin_ev[k & 1]));
+    const size_t off = k * half;
+    sink(off, h->pin_stage + (k & 1) * half, std::min(half, n - off));
+  }
+  return OTFEM_OK;
+}
+
+// Reads a [3N] node array (internal order) into caller order.
+otfem_status read_nodes3(otfem_solver *h, const double *src, double *dst) {
+  const int32_t *caller = h->L.node_of_internal.data();
+  return pinned_read(h, src, 3 * (size_t)h->L.N, [&](size_t off, const double *p, size_t len) {
+    for (size_t i = 0; i < len; i += 3) {
+      const int64_t u = caller[(off + i) / 3];
+      dst[3 * u] = p[i];
+      dst[3 * u + 1] = p[i + 1];
+      dst[3 * u + 2] = p[i + 2];
+    }
+  });
 }
 
 // Free-node ranks (internal order) for the sample store.
@@ -894,7 +959,7 @@ otfem_status otfem_set_external_forces(otfem_handle h, const double *f) {
   if (!h || !f) return OTFEM_INVALID_ARG;
   otfem_status st = ensure_device(h);
   if (st) return st;
-  std::vector<double> tmp;
+  std::vector<double> &tmp = h->stage;
   to_internal3(h->L, f, tmp);
   HIPCHK(h, hipMemcpyAsync(h->fext, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice,
                            h->stream));
@@ -906,19 +971,14 @@ otfem_status otfem_get_external_forces(otfem_handle h, double *f) {
   if (!h || !f) return OTFEM_INVALID_ARG;
   otfem_status st = ensure_device(h);
   if (st) return st;
-  std::vector<double> tmp(3 * h->L.N);
-  HIPCHK(h, hipMemcpyAsync(tmp.data(), h->fext, tmp.size() * sizeof(double), hipMemcpyDeviceToHost,
-                           h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  to_caller3(h->L, tmp, f);
-  return OTFEM_OK;
+  return read_nodes3(h, h->fext, f);
 }
 
 otfem_status otfem_set_state(otfem_handle h, const double *x, const double *v) {
   if (!h) return OTFEM_INVALID_ARG;
   otfem_status st = ensure_device(h);
   if (st) return st;
-  std::vector<double> tmp;
+  std::vector<double> &tmp = h->stage;
   if (x) {
     to_internal3(h->L, x, tmp);
     HIPCHK(h, hipMemcpyAsync(h->x, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice,
@@ -939,19 +999,8 @@ otfem_status otfem_get_state(otfem_handle h, double *x, double *v) {
   if (!h) return OTFEM_INVALID_ARG;
   otfem_status st = ensure_device(h);
   if (st) return st;
-  std::vector<double> tmp(3 * h->L.N);
-  if (x) {
-    HIPCHK(h, hipMemcpyAsync(tmp.data(), h->x, tmp.size() * sizeof(double), hipMemcpyDeviceToHost,
-                             h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    to_caller3(h->L, tmp, x);
-  }
-  if (v) {
-    HIPCHK(h, hipMemcpyAsync(tmp.data(), h->v, tmp.size() * sizeof(double), hipMemcpyDeviceToHost,
-                             h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    to_caller3(h->L, tmp, v);
-  }
+  if (x && (st = read_nodes3(h, h->x, x)) != OTFEM_OK) return st;
+  if (v && (st = read_nodes3(h, h->v, v)) != OTFEM_OK) return st;
   return OTFEM_OK;
 }
 
@@ -990,9 +1039,9 @@ otfem_status otfem_get_rotations(otfem_handle h, double *Re) {
       for (int i = 0; i < 9; ++i) Re[9 * t + i] = (i % 4 == 0) ? 1.0 : 0.0;
     return OTFEM_OK;
   }
-  HIPCHK(h, hipMemcpyAsync(Re, h->Re, 9 * h->L.T * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  return OTFEM_OK;
+  return pinned_read(h, h->Re, 9 * (size_t)h->L.T, [&](size_t off, const double *p, size_t len) {
+    std::memcpy(Re + off, p, len * sizeof(double));
+  });
 }
 
 otfem_status otfem_get_plastic(otfem_handle h, double *plastic) {
