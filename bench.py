@@ -434,6 +434,61 @@ def episode(s, w, args, models):
     return out
 
 
+def _wquantile(x, wts, q):
+    o = np.argsort(x)
+    c = np.cumsum(wts[o])
+    return float(x[o][min(len(x) - 1, np.searchsorted(c, q * c[-1]))])
+
+
+def start_residuals(s, w, args, models):
+    """The timed steps of the plain and learned legs again, synchronously:
+    per body and step the start residual |b - A v_start| / |b|, the final
+    one and the CG iterations, tet-weighted over bodies and steps.  CG's
+    iterations per decade of residual reduction belong to the operator; a
+    start only removes the decades between the plain start's residual and
+    its own, so at equal iterations per decade the start can cut CG
+    iterations by at most decades(plain) / decades(learned)."""
+    from opentissue_amd import fem
+    bt = (np.diff(w.body_tet_offsets) if w.num_bodies > 1 else np.array([w.num_tets])).astype(float)
+    out = {"steps": f"{args.warmup + 1}..{args.warmup + args.steps} from rest"}
+    for leg in ("plain", "learned"):
+        net = models.get("config") if leg == "learned" else None
+        P = fem.config_params(cap=args.cap, subspace=args.subspace if leg == "learned" else 0)
+        P.warm_start = fem.WARM_LEARNED if net is not None else fem.WARM_PREVIOUS_VELOCITY
+        s.set_warmstart_model(net)
+        s.set_state(w.coords, np.zeros_like(w.coords))
+        s.set_external_forces(w.forces)
+        r0, rf, it, wt = [], [], [], []
+        for k in range(args.warmup + args.steps):
+            s.simulate(w.dt, True, P)
+            if k >= args.warmup:
+                i, rel = s.body_stats()
+                r0.append(s.start_residuals())
+                rf.append(rel)
+                it.append(i.astype(float))
+                wt.append(bt)
+        s.set_warmstart_model(None)
+        r0, rf, it, wt = (np.concatenate(a) for a in (r0, rf, it, wt))
+        ok = (r0 > 0) & (rf > 0)
+        dec = np.where(ok, np.log10(np.maximum(r0, 1e-300) / np.maximum(rf, 1e-300)), 0.0)
+        out[leg] = {
+            "start_rel_residual_p10": _wquantile(r0, wt, 0.1),
+            "start_rel_residual_median": _wquantile(r0, wt, 0.5),
+            "start_rel_residual_p90": _wquantile(r0, wt, 0.9),
+            "final_rel_residual_median": _wquantile(rf, wt, 0.5),
+            "decades_per_step": float(np.sum(wt * dec) / wt.sum()),
+            "cg_iterations_per_step": float(np.sum(wt * it) / wt.sum()),
+            "iterations_per_decade": float(np.sum(wt * it) / max(np.sum(wt * dec), 1e-300)),
+        }
+        log(f"{w.name} start residuals {leg}: median {out[leg]['start_rel_residual_median']:.3g}, "
+            f"{out[leg]['decades_per_step']:.2f} decades and "
+            f"{out[leg]['cg_iterations_per_step']:.1f} iterations per step (tet-weighted)")
+    p, l = out["plain"], out["learned"]
+    out["iterations_ratio"] = p["cg_iterations_per_step"] / l["cg_iterations_per_step"]
+    out["decades_ratio"] = p["decades_per_step"] / l["decades_per_step"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -555,6 +610,7 @@ def main():
         result["cantilever_ab"] = ab
     if not args.no_episode and "learned" in legs:
         result["episode"] = episode(s, w, args, models)
+        result["start_residuals"] = start_residuals(s, w, args, models)
 
     if not args.no_cpu:
         if w.num_bodies > 1:

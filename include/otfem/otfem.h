@@ -234,6 +234,10 @@ otfem_status otfem_autotune_layout(const otfem_mesh_desc *desc, const double *yo
 const char *otfem_autotune_layout_last_error(void);
 /* Per-body CG iterations and final relative residual of the last step. */
 otfem_status otfem_body_stats(otfem_handle h, int *iterations, double *rel_residual);
+/* Per-body relative residual ||b - A v_start|| / ||b|| of the last step's CG
+ * start (after the warm start and any subspace start).  No reference
+ * counterpart (the reference reports no residuals). */
+otfem_status otfem_body_start_residuals(otfem_handle h, double *rel_residual);
 /* Coefficients c of the last subspace start, [B][12] (body-major; c[0] is
  * v^n's, then the differences from the newest, then the network's; unused
  * entries 0).  No reference counterpart (diagnostics of this start). */

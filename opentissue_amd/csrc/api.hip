@@ -1769,6 +1769,16 @@ otfem_status otfem_body_stats(otfem_handle h, int *iterations, double *rel_resid
   return OTFEM_OK;
 }
 
+otfem_status otfem_body_start_residuals(otfem_handle h, double *rel_residual) {
+  if (!h || !rel_residual) return OTFEM_INVALID_ARG;
+  if (h->last_body.size() != (size_t)h->L.B) return fail(h, OTFEM_STATE, "no completed step");
+  for (int64_t b = 0; b < h->L.B; ++b) {
+    const BodyState &s = h->last_body[b];
+    rel_residual[b] = s.bb > 0 ? std::sqrt(s.rr0 / s.bb) : 0.0;
+  }
+  return OTFEM_OK;
+}
+
 otfem_status otfem_subspace_coefficients(otfem_handle h, double *coef) {
   if (!h || !coef) return OTFEM_INVALID_ARG;
   if (!h->sub_coef) return fail(h, OTFEM_STATE, "no step with the subspace start yet");

@@ -151,6 +151,7 @@ SIGNATURES = {
     "otfem_autotune": (C.c_int, [_H, C.c_double, C.c_int, C.POINTER(Params), C.c_int,
                                  C.POINTER(C.c_int)]),
     "otfem_body_stats": (C.c_int, [_H, C.POINTER(C.c_int), _dp]),
+    "otfem_body_start_residuals": (C.c_int, [_H, _dp]),
     "otfem_update_original_coord": (C.c_int, [_H]),
     "otfem_export_K": (C.c_int, [_H, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                  C.POINTER(C.c_int32), _dp]),
@@ -472,6 +473,12 @@ class Solver:
         rel = np.empty(self.B)
         self._check(self.lib.otfem_body_stats(self.h, it.ctypes.data_as(C.POINTER(C.c_int)), _dptr(rel)))
         return it, rel
+
+    def start_residuals(self):
+        """Per body, ||b - A v_start|| / ||b|| of the last step's CG start."""
+        rel = np.empty(self.B)
+        self._check(self.lib.otfem_body_start_residuals(self.h, _dptr(rel)))
+        return rel
 
     def update_original_coord(self):
         self._check(self.lib.otfem_update_original_coord(self.h))
