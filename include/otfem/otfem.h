@@ -195,16 +195,17 @@ otfem_status otfem_unconverged_totals(otfem_handle h, int64_t *totals, int reset
  * counterpart (tuning of this implementation). */
 enum { OTFEM_SPMV_F32 = 0, OTFEM_SPMV_F32_ACC64 = 1, OTFEM_SPMV_F64 = 2, OTFEM_SPMV_F64_STORE64 = 3 };
 otfem_status otfem_set_kernel_config(otfem_handle h, int cg_block, int spmv_mode, int use_graphs);
-/* Deterministic mode (on = 1): every per-node sum of the step (elastic
- * forces, K' v_start, plastic forces, K' p in every CG iteration) is added
- * in a fixed order -- the node's slot terms in slot order -- instead of by
- * atomics in wave order, so a step from the same state, with the same
- * kernel configuration, gives the same bits on every run (the dot products
- * are fixed-order trees in both modes).  CG arithmetic is FP64 in this mode.
- * It costs a buffer of 288 B per slot and 3.8x the CG memory traffic: about
- * 5x the step time (64^3 cube: 28.7 M tet-steps/s against 143 M).  No
- * reference counterpart (the reference is single-threaded, hence
- * deterministic). */
+/* Deterministic mode (on = 1): every per-node sum of the step has a result
+ * that does not depend on the order of the atomics -- K' p in every CG
+ * iteration is summed as 64-bit fixed point (integer atomics, scale chosen
+ * per tile from a bound on the sums), the once-per-step sums (elastic
+ * forces, K' v_start, plastic forces, the subspace start's SpMV) add the
+ * node's slot terms in slot order -- so a step from the same state, with
+ * the same kernel configuration, gives the same bits on every run (the dot
+ * products are fixed-order trees in both modes).  CG arithmetic is FP64 in
+ * this mode.  It costs a buffer of 288 B per slot and about 1.1x the step
+ * time (64^3 cube: 8.92 ms against 8.12 ms).  No reference counterpart (the
+ * reference is single-threaded, hence deterministic). */
 otfem_status otfem_set_deterministic(otfem_handle h, int on);
 /* Autotuner: times `steps` steps with each CG block size (256/512/1024
  * threads per tile) on this mesh and these settings, restores the state

@@ -72,6 +72,7 @@ struct otfem_solver {
   int32_t *ord_begin = nullptr, *ord_ent = nullptr;
   double *ordered_terms = nullptr;
   double4 *tile_d = nullptr;  // [ntiles] D per unit volume and unit E of a tile's one nu
+  double *tile_kb = nullptr;  // [ntiles] bound of |K'p| / max|p| over the tile's owned nodes
   bool one_nu_per_tile = false;  // slot records may use the folded form
   double *Re = nullptr, *plastic = nullptr, *plastic_next = nullptr, *plastic_param = nullptr;
   BodyState *body = nullptr;
@@ -196,7 +197,7 @@ void free_device(otfem_solver *h) {
   dfree(h->x); dfree(h->x0); dfree(h->v); dfree(h->fext); dfree(h->mass);
   dfree(h->rhs); dfree(h->r); dfree(h->p); dfree(h->y); dfree(h->fixed);
   dfree(h->tiles); dfree(h->halo); dfree(h->conn); dfree(h->slot_tet);
-  dfree(h->rest64); dfree(h->vd64); dfree(h->bt4); dfree(h->bt8); dfree(h->tile_d);
+  dfree(h->rest64); dfree(h->vd64); dfree(h->bt4); dfree(h->bt8); dfree(h->tile_d); dfree(h->tile_kb);
   dfree(h->Re); dfree(h->plastic); dfree(h->plastic_next); dfree(h->plastic_param);
   dfree(h->body); dfree(h->part_bb); dfree(h->part_rr);
   dfree(h->part4); dfree(h->done_count); dfree(h->iter_acc); dfree(h->unconv_acc);
@@ -388,6 +389,7 @@ otfem_status ensure_device(otfem_solver *h) {
   HIPCHK(h, dalloc(&h->vd64, 3 * S));
   HIPCHK(h, dalloc(&h->bt4, 3 * S));
   HIPCHK(h, dalloc(&h->tile_d, nt));
+  HIPCHK(h, dalloc(&h->tile_kb, nt));
   HIPCHK(h, dalloc(&h->Re, 9 * T));
   HIPCHK(h, dalloc(&h->plastic, 6 * T));
   HIPCHK(h, dalloc(&h->plastic_next, 6 * T));
@@ -602,6 +604,36 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
       HIPCHK(h, hipMemcpyAsync(h->tile_d, td.data(), nt * sizeof(double4), hipMemcpyHostToDevice,
                                h->stream));
   }
+  // Bound of the deterministic CG's fixed-point node sums: per tile, the
+  // largest over owned nodes of sqrt(12) sum over the node's tets of
+  // trace(Ke) = sum_k (D0 + 2 D2) |b_k|^2 (D with |V|), since
+  // |(K'p)_i| <= sum_e ||Ke||_2 ||p_e||_2 and the rotation keeps the norm.
+  std::vector<double> kb(L.tiles.size(), 0.0);
+  {
+    std::vector<double> kn(N, 0.0);
+    for (int64_t t = 0; t < T; ++t) {
+      const double *gr = ed[t].grad;
+      double tr = 0.0;
+      for (int k = 0; k < 4; ++k) {
+        double n2 = 0.0;
+        for (int d = 0; d < 3; ++d) {
+          const double b = k == 0 ? -(gr[d] + gr[3 + d] + gr[6 + d]) : gr[3 * (k - 1) + d];
+          n2 += b * b;
+        }
+        tr += (ed[t].vd[0] + 2.0 * ed[t].vd[2]) * n2;
+      }
+      for (int k = 0; k < 4; ++k) kn[h->tets[4 * t + k]] += std::sqrt(12.0) * tr;
+    }
+    for (size_t i = 0; i < L.tiles.size(); ++i) {
+      const TileInfo &ti = L.tiles[i];
+      for (int l = 0; l < ti.n_owned; ++l)
+        kb[i] = std::max(kb[i], kn[L.node_of_internal[ti.node_begin + l]]);
+      kb[i] *= 1.0 + 1e-6;  // the slot records' FP32 rounding
+    }
+    if (!kb.empty())
+      HIPCHK(h, hipMemcpyAsync(h->tile_kb, kb.data(), kb.size() * sizeof(double),
+                               hipMemcpyHostToDevice, h->stream));
+  }
   std::vector<double> rest(9 * S), vd(3 * S);
   for (int64_t s = 0; s < S; ++s) {
     int64_t t = L.slot_tet[s] & 0x7fffffff;
@@ -673,6 +705,7 @@ StepArgs make_args(otfem_solver *h, double dt, int warp, const otfem_params &P) 
   a.plastic_on = h->plastic_on ? 1 : 0;
   a.store64 = (h->lc.spmv_mode == SPMV_F64_STORE64) ? 1 : 0;
   a.tile_d = h->tile_d;
+  a.tile_kb = h->tile_kb;
   a.folded = (h->one_nu_per_tile && h->lc.spmv_mode == SPMV_F64) ? 1 : 0;
   a.ordered = h->ordered ? 1 : 0;
   a.ord_begin = h->ord_begin;

@@ -55,6 +55,7 @@ struct StepArgs {
                          //         copy and committed to `plastic` after k_element
   const double *plastic_param;  // [T][3] yield, creep, max
   const double4 *tile_d; // [ntiles] (D0, D1, D2, 0) per unit volume and unit E of a tile's one nu
+  const double *tile_kb; // [ntiles] max over owned nodes of sqrt(12) sum of trace(Ke) of its tets
   // bodies / reductions (no atomics except done_count)
   BodyState *body;
   double *part_bb;       // [ntiles]  b.b (element kernel)

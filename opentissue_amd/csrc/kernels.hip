@@ -711,6 +711,15 @@ __device__ __forceinline__ void slot_spmv_store(ushort4 c, const MT bt[3][3], co
 // Accumulator tag of the deterministic mode (K'p summed per node in slot
 // order, in FP64); AccOf gives the LDS type of the node sums.
 struct Ordered {};
+// Accumulator tag of the deterministic CG: K'p summed per node in LDS as
+// 64-bit fixed point.  Integer addition is exact and associative, so the sum
+// does not depend on the order of the LDS atomics.  The tile's scale 2^k is
+// chosen from a bound on every node sum, |(K'p)_i| <= kb * max|p| with
+// kb = max over owned nodes of sqrt(12) sum trace(Ke) (||K'_e||_2 = ||Ke||_2 <=
+// trace(Ke), ||p_e||_2 <= sqrt(12) max|p|), so that terms and sums stay below
+// 2^50; a term is rounded to a multiple of 2^-k, an absolute error of
+// 2^-51 of that bound.
+struct Fixed {};
 template <typename AT>
 struct AccOf {
   using type = AT;
@@ -719,6 +728,46 @@ template <>
 struct AccOf<Ordered> {
   using type = double;
 };
+template <>
+struct AccOf<Fixed> {
+  using type = unsigned long long;
+};
+
+// Node sum in LDS -> FP64 (inv = 2^-k of the fixed-point sums).
+template <typename T>
+__device__ __forceinline__ double acc_value(T v, double) {
+  return (double)v;
+}
+template <>
+__device__ __forceinline__ double acc_value<unsigned long long>(unsigned long long v, double inv) {
+  return (double)(long long)v * inv;
+}
+
+// round(x sc) as a two's-complement integer, for |x sc| < 2^51: adding
+// 1.5 * 2^52 leaves the rounded value in the low mantissa bits (the unit in
+// the last place is 1 there), with one rounding in the fused multiply-add.
+__device__ __forceinline__ unsigned long long to_fixed(double x, double sc) {
+  constexpr double kMagic = 6755399441055744.0;  // 1.5 * 2^52
+  return (unsigned long long)(__double_as_longlong(__fma_rn(x, sc, kMagic)) -
+                              __double_as_longlong(kMagic));
+}
+
+// y_owned += (K' p)_e for one slot as fixed point (LDS integer atomics).
+template <typename MT>
+__device__ __forceinline__ void slot_spmv_fixed(ushort4 c, const MT bt[3][3], const MT D[3],
+                                                const double *sp, unsigned long long *sy, int no,
+                                                double sc) {
+  MT f[4][3];
+  slot_forces<MT>(c, bt, D, sp, f);
+  const int cc[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (cc[j] < no) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) atomicAdd(&sy[3 * cc[j] + d], to_fixed((double)f[j][d], sc));
+    }
+  }
+}
 
 }  // namespace
 
@@ -794,7 +843,7 @@ template <typename AT>
 __device__ __forceinline__ void update_node(const StepArgs &A, int it, bool stop, double alpha,
                                             double beta, int l, int no, const NodeIn &n,
                                             double *Rn, double *Pn, double *sp, double *sr,
-                                            AT *sy, double *sm) {
+                                            AT *sy, double *sm, double &pmax) {
   const int64_t g = n.g;
   double rn[3], pn[3];
   if (n.fx) {
@@ -816,7 +865,11 @@ __device__ __forceinline__ void update_node(const StepArgs &A, int it, bool stop
     }
   }
 #pragma unroll
-  for (int d = 0; d < 3; ++d) sp[3 * l + d] = pn[d];
+  for (int d = 0; d < 3; ++d) {
+    sp[3 * l + d] = pn[d];
+    // a non-finite p makes the bound infinite (max ignores NaN)
+    pmax = isfinite(pn[d]) ? fmax(pmax, fabs(pn[d])) : HUGE_VAL;
+  }
   if (l < no) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -834,10 +887,10 @@ __device__ __forceinline__ void update_node(const StepArgs &A, int it, bool stop
 template <typename AT>
 __device__ __forceinline__ void node_partials(const StepArgs &A, int64_t g, int l, double m,
                                               const double *sp, const double *sr, const AT *sy,
-                                              double *Yn, double4 &acc) {
+                                              double *Yn, double4 &acc, double inv) {
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const double yd = (double)sy[3 * l + d];
+    const double yd = acc_value<AT>(sy[3 * l + d], inv);
     Yn[3 * g + d] = yd;
     const double p = sp[3 * l + d], r = sr[3 * l + d];
     const double q = A.cM * m * p + A.cK * yd;
@@ -871,7 +924,9 @@ void k_cg_iter(StepArgs A, int it) {
   constexpr int NB = CgShape<BLOCK>::kNodes;
   constexpr int U = sizeof(ST) == 8 ? (CgShape<BLOCK>::kSlots + 1) / 2 : CgShape<BLOCK>::kSlots;
   constexpr bool kOrdered = std::is_same<AT, Ordered>::value;
+  constexpr bool kFixed = std::is_same<AT, Fixed>::value;
   using Acc = typename AccOf<AT>::type;
+  __shared__ unsigned long long s_pmax;  // max |p| over the tile (FP64 bits), fixed point only
   int tile;
   if (A.compact) {
     // workgroup -> active tile; the remap keeps neighbouring tiles on one XCD
@@ -940,6 +995,7 @@ void k_cg_iter(StepArgs A, int it) {
     part.z += q.z;
     part.w += q.w;
   }
+  if (kFixed && threadIdx.x == 0) s_pmax = 0ull;  // ordered by block_sum4's barriers
   const double4 s4 = block_sum4<BLOCK>(part, red);
   double alpha = 0.0, beta = 0.0, rr, bb;
   if (it == 0) {
@@ -980,18 +1036,34 @@ void k_cg_iter(StepArgs A, int it) {
 
   // prologue: finish iteration it-1 and form the new direction, NB nodes per
   // thread at a time (the first batch was loaded above)
+  double pmax = 0.0;
   for (int base = 0;;) {
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
       const int l = base + (int)threadIdx.x + k * BLOCK;
-      if (l < nn) update_node<Acc>(A, it, stop, alpha, beta, l, no, nd[k], Rn, Pn, sp, sr, sy, sm);
+      if (l < nn)
+        update_node<Acc>(A, it, stop, alpha, beta, l, no, nd[k], Rn, Pn, sp, sr, sy, sm, pmax);
     }
     base += NB * BLOCK;
     if (base >= nn) break;
     load_nodes<BLOCK, NB>(A, ti, it, base, Rp, Pp, Yp, nd);
   }
   if (stop) return;
+  if (kFixed && pmax > 0.0) atomicMax(&s_pmax, (unsigned long long)__double_as_longlong(pmax));
   __syncthreads();
+  // fixed-point scale 2^k with kb max|p| 2^k < 2^50 (inv = 2^-k; NaN when the
+  // bound is not finite, so the node sums come out non-finite)
+  double sc = 1.0, inv = 1.0;
+  if constexpr (kFixed) {
+    const double bound = A.tile_kb[tile] * __longlong_as_double((long long)s_pmax);
+    if (!isfinite(bound)) {
+      inv = __longlong_as_double(0x7ff8000000000000ll);
+    } else if (bound > 0.0) {
+      const int k = min(50 - (ilogb(bound) + 1), 1000);
+      sc = ldexp(1.0, k);
+      inv = ldexp(1.0, -k);
+    }
+  }
 
   // q = K' p over the tile's slots: buffer ra is in use while rb loads, and
   // the other way round
@@ -1006,6 +1078,8 @@ void k_cg_iter(StepArgs A, int it) {
           unpack_raw<MT, ST>(ra[u], Dt, bt, D);
           if constexpr (kOrdered)
             slot_spmv_store<MT>(ra[u].c, bt, D, sp, A.ordered_terms + 12 * (int64_t)(s + u * BLOCK));
+          else if constexpr (kFixed)
+            slot_spmv_fixed<MT>(ra[u].c, bt, D, sp, sy, no, sc);
           else
             slot_spmv<MT, AT>(ra[u].c, bt, D, sp, sy, no);
         }
@@ -1022,6 +1096,8 @@ void k_cg_iter(StepArgs A, int it) {
           if constexpr (kOrdered)
             slot_spmv_store<MT>(rb[u].c, bt, D, sp,
                                 A.ordered_terms + 12 * (int64_t)(s + (U + u) * BLOCK));
+          else if constexpr (kFixed)
+            slot_spmv_fixed<MT>(rb[u].c, bt, D, sp, sy, no, sc);
           else
             slot_spmv<MT, AT>(rb[u].c, bt, D, sp, sy, no);
         }
@@ -1049,7 +1125,7 @@ void k_cg_iter(StepArgs A, int it) {
       sy[3 * l + 1] = y1;
       sy[3 * l + 2] = y2;
     }
-    if (m >= 0.0) node_partials<Acc>(A, ti.node_begin + l, l, m, sp, sr, sy, Yn, acc);
+    if (m >= 0.0) node_partials<Acc>(A, ti.node_begin + l, l, m, sp, sr, sy, Yn, acc, inv);
   }
   const double4 tot = block_sum4<BLOCK>(acc, red);
   if (threadIdx.x == 0) reinterpret_cast<double4 *>(A.part4 + cur * 4 * (int64_t)nt)[tile] = tot;
@@ -1667,11 +1743,11 @@ __global__ __launch_bounds__(kCompactBlock) void k_compact(StepArgs A) {
 template <int BLOCK>
 hipError_t enqueue_mode(const StepArgs &a, int mode, size_t lds_cg, hipStream_t s, int it0,
                         int it1, int g) {
-  if (a.ordered) {  // deterministic mode: FP64 arithmetic, ordered node sums
+  if (a.ordered) {  // deterministic mode: FP64 arithmetic, fixed-point node sums
     if (mode == SPMV_F64_STORE64)
-      return enqueue_iterations<BLOCK, double, Ordered, double>(a, lds_cg, s, it0, it1, g);
-    if (a.folded) return enqueue_iterations<BLOCK, double, Ordered, Folded>(a, lds_cg, s, it0, it1, g);
-    return enqueue_iterations<BLOCK, double, Ordered, float>(a, lds_cg, s, it0, it1, g);
+      return enqueue_iterations<BLOCK, double, Fixed, double>(a, lds_cg, s, it0, it1, g);
+    if (a.folded) return enqueue_iterations<BLOCK, double, Fixed, Folded>(a, lds_cg, s, it0, it1, g);
+    return enqueue_iterations<BLOCK, double, Fixed, float>(a, lds_cg, s, it0, it1, g);
   }
   switch (mode) {
     case SPMV_F32: return enqueue_iterations<BLOCK, float, float, float>(a, lds_cg, s, it0, it1, g);
