@@ -331,11 +331,16 @@ def setup_solver(w, P, tile_nodes, cg_block=0):
     t1 = time.perf_counter()
     s = fem.Solver(w.coords, w.tets, w.fixed, body_node_offsets=w.body_node_offsets,
                    body_tet_offsets=w.body_tet_offsets, target_tile_nodes=tile_nodes)
-    log(f"otfem_create (tile layout) {time.perf_counter() - t1:.2f} s")
+    create_s = time.perf_counter() - t1
+    log(f"otfem_create (tile layout) {create_s:.2f} s")
+    t1 = time.perf_counter()
     s.init_materials(w.young, w.poisson, w.density)
+    init_s = time.perf_counter() - t1
+    log(f"otfem_init_materials (element data, masses, bounds on the host) {init_s:.2f} s")
     s.set_external_forces(w.forces)
     block = cg_block or tuned_block or s.autotune(w.dt, True, P, steps=2)
     s.set_kernel_config(cg_block=block)
+    s.setup_seconds = {"otfem_create": create_s, "otfem_init_materials": init_s}
     return s, tile_nodes, block, tile_nodes < 0
 
 
@@ -350,7 +355,8 @@ def measure(w, args, models, tile_nodes, legs=("learned", "plain", "subspace", "
     if args.deterministic:
         s.set_deterministic(True)
     layout = s.layout_info()
-    out = {"tile_nodes": tile_nodes, "cg_block": block, "layout": layout}
+    out = {"tile_nodes": tile_nodes, "cg_block": block, "layout": layout,
+           "setup_seconds": s.setup_seconds}
     for leg in legs:
         net = models.get("config") if leg in ("learned", "network") else None
         sub = args.subspace if leg in ("learned", "subspace") else 0
@@ -584,6 +590,7 @@ def main():
             "tiles": m["layout"]["tiles"], "slots": m["layout"]["slots"], "cg_block": m["cg_block"],
             "tile_nodes": m["tile_nodes"], "layout_autotuned": args.tile_nodes < 0,
             "deterministic": bool(args.deterministic),
+            "setup_seconds": m["setup_seconds"],
         },
         "leg": legs[0],
         "learned_vs_plain": (m["learned"]["value"] / m["plain"]["value"]

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -390,6 +391,7 @@ otfem_status ensure_device(otfem_solver *h) {
   HIPCHK(h, dalloc(&h->bt4, 3 * S));
   HIPCHK(h, dalloc(&h->tile_d, nt));
   HIPCHK(h, dalloc(&h->tile_kb, nt));
+  HIPCHK(h, dalloc(&h->body_tiles, B + 1));
   HIPCHK(h, dalloc(&h->Re, 9 * T));
   HIPCHK(h, dalloc(&h->plastic, 6 * T));
   HIPCHK(h, dalloc(&h->plastic_next, 6 * T));
@@ -404,6 +406,8 @@ otfem_status ensure_device(otfem_solver *h) {
   HIPCHK(h, dalloc(&h->unconv_acc, B));
   hipStream_t s = h->stream;
   HIPCHK(h, hipMemcpyAsync(h->tiles, L.tiles.data(), nt * sizeof(TileInfo), hipMemcpyHostToDevice, s));
+  HIPCHK(h, hipMemcpyAsync(h->body_tiles, L.body_tile_begin.data(), (B + 1) * sizeof(int32_t),
+                           hipMemcpyHostToDevice, s));
   if (!L.halo.empty())
     HIPCHK(h, hipMemcpyAsync(h->halo, L.halo.data(), L.halo.size() * sizeof(int32_t),
                              hipMemcpyHostToDevice, s));
@@ -497,38 +501,84 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
     return fail(h, OTFEM_INVALID_ARG, "plastic parameters must be non-negative");
   // everything is checked before the solver changes: a rejected init leaves
   // the previous materials, element data and masses in place
+  // The loops below run on host threads, in chunks of tets, slots or nodes;
+  // per-node sums run per body (a body's tets touch only its own nodes), each
+  // in tet order, so every value is the one a single thread computes.
+  const int nth = host_threads();
+  constexpr int64_t kChunk = int64_t(1) << 16;
+  auto par = [&](int64_t n, auto &&f) {  // f(i0, i1) over [0, n)
+    const int64_t nc = (n + kChunk - 1) / kChunk;
+    parallel_for(nc, nth, [&](int64_t c) { f(c * kChunk, std::min(n, (c + 1) * kChunk)); });
+  };
+  const int64_t nb = std::max<int64_t>(L.B, 1);
+  bool by_body = L.B > 1 && (int64_t)L.body_tet_begin.size() == L.B + 1 &&
+                 (int64_t)L.body_node_begin.size() == L.B + 1;
+  if (by_body) {
+    std::atomic<bool> inside{true};
+    parallel_for(nb, nth, [&](int64_t b) {
+      const int64_t n0 = L.body_node_begin[b], n1 = L.body_node_begin[b + 1];
+      for (int64_t t = L.body_tet_begin[b]; t < L.body_tet_begin[b + 1]; ++t)
+        for (int k = 0; k < 4; ++k) {
+          const int64_t g = h->tets[4 * t + k];
+          if (g < n0 || g >= n1) inside.store(false, std::memory_order_relaxed);
+        }
+    });
+    by_body = inside.load();
+  }
+  // per_node(f): f(t) for every tet, in tet order within each body (or all
+  // tets in order on one thread), for sums into the tet's nodes
+  auto per_node = [&](auto &&f) {
+    if (!by_body) {
+      for (int64_t t = 0; t < T; ++t) f(t);
+      return;
+    }
+    parallel_for(nb, nth, [&](int64_t b) {
+      for (int64_t t = L.body_tet_begin[b]; t < L.body_tet_begin[b + 1]; ++t) f(t);
+    });
+  };
   std::vector<double> young(T), poisson(T), density(T);
-  for (int64_t t = 0; t < T; ++t) {
-    double e = uniform ? E[0] : E[t], n = uniform ? nu[0] : nu[t], r = uniform ? rho[0] : rho[t];
-    // fem_uniform_young.h:32, fem_uniform_poisson.h:32-33, density > 0; nu = 0.5
-    // is rejected as well since it makes D infinite.
-    if (!(e >= 0) || !(n >= 0) || !(n < 0.5) || !(r > 0) || !std::isfinite(e) || !std::isfinite(r))
-      return fail(h, OTFEM_INVALID_ARG, "material out of range (E>=0, 0<=nu<0.5, rho>0)");
-    young[t] = e;
-    poisson[t] = n;
-    density[t] = r;
-  }
-  std::vector<ElemData> ed(T);
+  std::atomic<bool> bad_material{false};
+  par(T, [&](int64_t t0, int64_t t1) {
+    for (int64_t t = t0; t < t1; ++t) {
+      double e = uniform ? E[0] : E[t], n = uniform ? nu[0] : nu[t], r = uniform ? rho[0] : rho[t];
+      // fem_uniform_young.h:32, fem_uniform_poisson.h:32-33, density > 0; nu = 0.5
+      // is rejected as well since it makes D infinite.
+      if (!(e >= 0) || !(n >= 0) || !(n < 0.5) || !(r > 0) || !std::isfinite(e) || !std::isfinite(r))
+        bad_material.store(true, std::memory_order_relaxed);
+      young[t] = e;
+      poisson[t] = n;
+      density[t] = r;
+    }
+  });
+  if (bad_material.load())
+    return fail(h, OTFEM_INVALID_ARG, "material out of range (E>=0, 0<=nu<0.5, rho>0)");
+  // 104 B per tet (6.1 GB on 59 M tets): left uninitialised, every entry is
+  // written by the thread that computes it
+  std::unique_ptr<ElemData[]> ed(new ElemData[std::max<int64_t>(T, 1)]);
   std::vector<double> vol(T);
-  int64_t inverted = 0;
+  std::atomic<int64_t> inverted{0};
   const double *x0 = h->x0_host.data();
-  for (int64_t t = 0; t < T; ++t) {
-    const int32_t *tt = &h->tets[4 * t];
-    ed[t] = element_data(x0 + 3 * tt[0], x0 + 3 * tt[1], x0 + 3 * tt[2], x0 + 3 * tt[3], young[t],
-                         poisson[t]);
-    vol[t] = ed[t].V;
-    if (!(ed[t].V > 0)) ++inverted;
-  }
-  if (inverted)
+  par(T, [&](int64_t a0, int64_t a1) {
+    int64_t inv = 0;
+    for (int64_t t = a0; t < a1; ++t) {
+      const int32_t *tt = &h->tets[4 * t];
+      ed[t] = element_data(x0 + 3 * tt[0], x0 + 3 * tt[1], x0 + 3 * tt[2], x0 + 3 * tt[3],
+                           young[t], poisson[t]);
+      vol[t] = ed[t].V;
+      if (!(ed[t].V > 0)) ++inv;
+    }
+    inverted.fetch_add(inv, std::memory_order_relaxed);
+  });
+  if (inverted.load())
     return fail(h, OTFEM_INVERTED_TET,
-                std::to_string(inverted) + " tetrahedra have non-positive rest volume");
+                std::to_string(inverted.load()) + " tetrahedra have non-positive rest volume");
   // fem_compute_mass.h:154-171 in caller order
   std::vector<double> mass(N, 0.0);
   for (int64_t i = 0; i < N; ++i) mass[i] = h->fixed_host[i] ? std::numeric_limits<double>::max() : 0.0;
-  for (int64_t t = 0; t < T; ++t) {
+  per_node([&](int64_t t) {
     double amount = density[t] * ed[t].V * .25;
     for (int k = 0; k < 4; ++k) mass[h->tets[4 * t + k]] += amount;
-  }
+  });
   // A free node outside every tetrahedron has zero mass and an empty row of A;
   // the reference would divide by zero in CG.  Reject it (fix or drop it).
   for (int64_t i = 0; i < N; ++i)
@@ -559,7 +609,7 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
   // with D = V (d16, d17, d18)); feature of the learned warm start.
   {
     std::vector<double> kdc(3 * N, 0.0);
-    for (int64_t t = 0; t < T; ++t) {
+    per_node([&](int64_t t) {
       const double *gr = ed[t].grad;
       double b[4][3];
       for (int d = 0; d < 3; ++d) {
@@ -574,10 +624,12 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
           kdc[3 * (int64_t)h->tets[4 * t + k] + d] +=
               ed[t].vd[0] * b[k][d] * b[k][d] + ed[t].vd[2] * (n2 - b[k][d] * b[k][d]);
       }
-    }
+    });
     h->kd_host.assign(4 * N, 0.0f);
-    for (int64_t g = 0; g < N; ++g)
-      for (int d = 0; d < 3; ++d) h->kd_host[4 * g + d] = (float)kdc[3 * L.node_of_internal[g] + d];
+    par(N, [&](int64_t g0, int64_t g1) {
+      for (int64_t g = g0; g < g1; ++g)
+        for (int d = 0; d < 3; ++d) h->kd_host[4 * g + d] = (float)kdc[3 * L.node_of_internal[g] + d];
+    });
   }
   // One Poisson ratio per tile: D of a tet is then E |V| times constants of
   // the tile, so the CG's slot records fold sqrt(E |V|) into the gradients
@@ -586,20 +638,23 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
   std::vector<double4> td(L.tiles.size(), make_double4(0.0, 0.0, 0.0, 0.0));
   {
     const int64_t nt = (int64_t)L.tiles.size();
-    bool one = true;
-    for (int64_t i = 0; i < nt && one; ++i) {
+    std::atomic<bool> one{true};
+    parallel_for(nt, nth, [&](int64_t i) {
       const TileInfo &ti = L.tiles[i];
-      if (ti.n_slots == 0) continue;
+      if (ti.n_slots == 0) return;
       const int64_t t0 = L.slot_tet[ti.slot_begin] & 0x7fffffff;
-      for (int64_t s = ti.slot_begin; s < ti.slot_begin + ti.n_slots && one; ++s) {
+      for (int64_t s = ti.slot_begin; s < ti.slot_begin + ti.n_slots; ++s) {
         const int64_t t = L.slot_tet[s] & 0x7fffffff;
-        one = h->poisson[t] == h->poisson[t0];
+        if (h->poisson[t] != h->poisson[t0]) {
+          one.store(false, std::memory_order_relaxed);
+          break;
+        }
       }
       double dm[3];
       material_d(1.0, h->poisson[t0], dm);
       td[i] = make_double4(dm[0], dm[1], dm[2], 0.0);
-    }
-    h->one_nu_per_tile = one;
+    });
+    h->one_nu_per_tile = one.load();
     if (nt > 0)
       HIPCHK(h, hipMemcpyAsync(h->tile_d, td.data(), nt * sizeof(double4), hipMemcpyHostToDevice,
                                h->stream));
@@ -611,7 +666,7 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
   std::vector<double> kb(L.tiles.size(), 0.0);
   {
     std::vector<double> kn(N, 0.0);
-    for (int64_t t = 0; t < T; ++t) {
+    per_node([&](int64_t t) {
       const double *gr = ed[t].grad;
       double tr = 0.0;
       for (int k = 0; k < 4; ++k) {
@@ -623,38 +678,46 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
         tr += (ed[t].vd[0] + 2.0 * ed[t].vd[2]) * n2;
       }
       for (int k = 0; k < 4; ++k) kn[h->tets[4 * t + k]] += std::sqrt(12.0) * tr;
-    }
-    for (size_t i = 0; i < L.tiles.size(); ++i) {
+    });
+    parallel_for((int64_t)L.tiles.size(), nth, [&](int64_t i) {
       const TileInfo &ti = L.tiles[i];
       for (int l = 0; l < ti.n_owned; ++l)
         kb[i] = std::max(kb[i], kn[L.node_of_internal[ti.node_begin + l]]);
       kb[i] *= 1.0 + 1e-6;  // the slot records' FP32 rounding
-    }
+    });
     if (!kb.empty())
       HIPCHK(h, hipMemcpyAsync(h->tile_kb, kb.data(), kb.size() * sizeof(double),
                                hipMemcpyHostToDevice, h->stream));
   }
-  std::vector<double> rest(9 * S), vd(3 * S);
-  for (int64_t s = 0; s < S; ++s) {
-    int64_t t = L.slot_tet[s] & 0x7fffffff;
-    for (int k = 0; k < 9; ++k) rest[k * S + s] = ed[t].grad[k];
-    for (int k = 0; k < 3; ++k) vd[k * S + s] = ed[t].vd[k];
-  }
+  // per-slot element data (SoA), filled by the threads that first touch it
+  std::unique_ptr<double[]> rest(new double[std::max<int64_t>(9 * S, 1)]);
+  std::unique_ptr<double[]> vd(new double[std::max<int64_t>(3 * S, 1)]);
+  par(S, [&](int64_t s0, int64_t s1) {
+    for (int64_t s = s0; s < s1; ++s) {
+      int64_t t = L.slot_tet[s] & 0x7fffffff;
+      for (int k = 0; k < 9; ++k) rest[k * S + s] = ed[t].grad[k];
+      for (int k = 0; k < 3; ++k) vd[k * S + s] = ed[t].vd[k];
+    }
+  });
   std::vector<double> mint(N), pp(3 * T);
-  for (int64_t g = 0; g < N; ++g) mint[g] = h->mass_host[L.node_of_internal[g]];
-  for (int64_t t = 0; t < T; ++t) {
-    pp[3 * t] = c_yield;
-    pp[3 * t + 1] = c_creep;
-    pp[3 * t + 2] = c_max;
-  }
+  par(N, [&](int64_t g0, int64_t g1) {
+    for (int64_t g = g0; g < g1; ++g) mint[g] = h->mass_host[L.node_of_internal[g]];
+  });
+  par(T, [&](int64_t t0, int64_t t1) {
+    for (int64_t t = t0; t < t1; ++t) {
+      pp[3 * t] = c_yield;
+      pp[3 * t + 1] = c_creep;
+      pp[3 * t + 2] = c_max;
+    }
+  });
   h->plastic_on = (c_creep > 0.0 && c_max > 0.0);
   h->plastic_c[0] = c_yield;
   h->plastic_c[1] = c_creep;
   h->plastic_c[2] = c_max;
   hipStream_t s = h->stream;
   if (S > 0) {
-    HIPCHK(h, hipMemcpyAsync(h->rest64, rest.data(), 9 * S * sizeof(double), hipMemcpyHostToDevice, s));
-    HIPCHK(h, hipMemcpyAsync(h->vd64, vd.data(), 3 * S * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(h, hipMemcpyAsync(h->rest64, rest.get(), 9 * S * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(h, hipMemcpyAsync(h->vd64, vd.get(), 3 * S * sizeof(double), hipMemcpyHostToDevice, s));
   }
   HIPCHK(h, hipMemcpyAsync(h->mass, mint.data(), N * sizeof(double), hipMemcpyHostToDevice, s));
   if (T > 0)
@@ -735,11 +798,6 @@ otfem_status sub_ensure(otfem_solver *h, int hist, int directions) {
   if (!h->sub_coef) {
     HIPCHK(h, dalloc(&h->sub_coef, kSubMax * B));
     HIPCHK(h, hipMemsetAsync(h->sub_coef, 0, kSubMax * B * sizeof(double), h->stream));
-  }
-  if (!h->body_tiles) {
-    HIPCHK(h, dalloc(&h->body_tiles, B + 1));
-    HIPCHK(h, hipMemcpyAsync(h->body_tiles, h->L.body_tile_begin.data(), (B + 1) * sizeof(int32_t),
-                             hipMemcpyHostToDevice, h->stream));
   }
   if (!h->sub_state) {
     HIPCHK(h, dalloc(&h->sub_state, 3));

@@ -1718,13 +1718,16 @@ hipError_t enqueue_iterations(const StepArgs &a, size_t lds_cg, hipStream_t s, i
 // contiguous range of tiles.
 constexpr int kCompactBlock = 1024;
 __global__ __launch_bounds__(kCompactBlock) void k_compact(StepArgs A) {
+  // by body: a running body contributes its contiguous tile range, so each
+  // thread reads one stop flag per body (not one per tile), and the list
+  // comes out in tile order
   __shared__ int scan[kCompactBlock];
-  const int nt = A.ntiles;
-  const int per = (nt + kCompactBlock - 1) / kCompactBlock;
-  const int t0 = threadIdx.x * per, t1 = min(nt, t0 + per);
+  const int nb = A.B;
+  const int per = (nb + kCompactBlock - 1) / kCompactBlock;
+  const int b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
   int cnt = 0;
-  for (int t = t0; t < t1; ++t)
-    cnt += *(volatile int32_t *)&A.body[A.tiles[t].body].stopped ? 0 : 1;
+  for (int b = b0; b < b1; ++b)
+    if (!*(volatile int32_t *)&A.body[b].stopped) cnt += A.body_tiles[b + 1] - A.body_tiles[b];
   scan[threadIdx.x] = cnt;
   __syncthreads();
   // inclusive Hillis-Steele scan
@@ -1735,8 +1738,9 @@ __global__ __launch_bounds__(kCompactBlock) void k_compact(StepArgs A) {
     __syncthreads();
   }
   int pos = scan[threadIdx.x] - cnt;
-  for (int t = t0; t < t1; ++t)
-    if (!*(volatile int32_t *)&A.body[A.tiles[t].body].stopped) A.act[pos++] = t;
+  for (int b = b0; b < b1; ++b)
+    if (!*(volatile int32_t *)&A.body[b].stopped)
+      for (int t = A.body_tiles[b]; t < A.body_tiles[b + 1]; ++t) A.act[pos++] = t;
   if (threadIdx.x == kCompactBlock - 1) A.done_count[1] = scan[kCompactBlock - 1];
 }
 

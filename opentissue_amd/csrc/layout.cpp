@@ -50,38 +50,6 @@ void rcb(std::vector<int32_t> &idx, int64_t lo, int64_t hi, int64_t parts, const
   rcb(idx, mid, hi, parts - left_parts, x, w, part_sizes);
 }
 
-// Runs f(i) for i in [0, n) on up to `threads` threads (the caller's thread
-// is one of them); items are handed out one at a time, so uneven bodies and
-// tiles balance.  Each f(i) must write only to data that item owns.  An
-// exception in any item (a failed allocation) stops the hand-out and is
-// rethrown on the calling thread once every worker has joined.
-template <class F>
-void parallel_for(int64_t n, int threads, F &&f) {
-  const int64_t nt = std::min<int64_t>(threads, n);
-  if (nt <= 1) {
-    for (int64_t i = 0; i < n; ++i) f(i);
-    return;
-  }
-  std::atomic<int64_t> next{0};
-  std::exception_ptr error;
-  std::mutex error_lock;
-  auto work = [&] {
-    try {
-      for (int64_t i; (i = next.fetch_add(1, std::memory_order_relaxed)) < n;) f(i);
-    } catch (...) {
-      std::lock_guard<std::mutex> g(error_lock);
-      if (!error) error = std::current_exception();
-      next.store(n);
-    }
-  };
-  std::vector<std::thread> pool;
-  pool.reserve(nt - 1);
-  for (int64_t t = 1; t < nt; ++t) pool.emplace_back(work);
-  work();
-  for (auto &t : pool) t.join();
-  if (error) std::rethrow_exception(error);
-}
-
 // Picks an order for the slots of one tile, in windows of 64 (one wave
 // instruction) in which no two slots share an owned corner, so the per-corner
 // LDS atomics of a wave never hit the same address.  Each quarter window (16

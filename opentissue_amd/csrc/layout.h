@@ -13,11 +13,54 @@
 //     axpys) fuse into the same pass.  The node sums are LDS atomics, so
 //     their rounding depends on wave order (not bitwise reproducible).
 #pragma once
+#include <algorithm>
+#include <atomic>
 #include <cstdint>
+#include <exception>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace otfem {
+
+// Runs f(i) for i in [0, n) on up to `threads` threads (the caller's thread
+// is one of them); items are handed out one at a time, so uneven bodies and
+// tiles balance.  Each f(i) must write only to data that item owns.  An
+// exception in any item (a failed allocation) stops the hand-out and is
+// rethrown on the calling thread once every worker has joined.
+template <class F>
+void parallel_for(int64_t n, int threads, F &&f) {
+  const int64_t nt = std::min<int64_t>(threads, n);
+  if (nt <= 1) {
+    for (int64_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<int64_t> next{0};
+  std::exception_ptr error;
+  std::mutex error_lock;
+  auto work = [&] {
+    try {
+      for (int64_t i; (i = next.fetch_add(1, std::memory_order_relaxed)) < n;) f(i);
+    } catch (...) {
+      std::lock_guard<std::mutex> g(error_lock);
+      if (!error) error = std::current_exception();
+      next.store(n);
+    }
+  };
+  std::vector<std::thread> pool;
+  pool.reserve(nt - 1);
+  for (int64_t t = 1; t < nt; ++t) pool.emplace_back(work);
+  work();
+  for (auto &t : pool) t.join();
+  if (error) std::rethrow_exception(error);
+}
+
+// Host threads for set-up work: one per hardware thread, at most 16.
+inline int host_threads() {
+  return (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+}
+
 
 struct TileInfo {
   int32_t node_begin;  // first owned internal node id
