@@ -412,6 +412,11 @@ def roofline(w, leg, profile_key):
                                "size; not measured in this run")
         if "avg_launch_us" in out:
             out["counted_frac"] = traffic / (out["avg_launch_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS
+        elif profile_avg_us:
+            # batched: bytes and duration of a full-grid launch, both from the profiles
+            out["counted_frac"] = traffic / (profile_avg_us * 1e-6) / 1e9 / HBM_PEAK_GBS
+            out["counted_note"] = ("full-grid k_cg_iter launches: counted HBM bytes over the kernel "
+                                   "trace's mean duration, both from the committed profiles")
     if profile_avg_us:
         out["profile_avg_launch_us"] = profile_avg_us
         out["profile_note"] = "committed rocprofv3 kernel trace (profiles/), not measured in this run"
