@@ -61,10 +61,17 @@ __device__ __forceinline__ int unit_of(int t, int r, int h) {
   return 32 * t + 8 * (r >> 2) + 4 * h + (r & 3);
 }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// The logistic 1 / (1 + e^-z) with the hardware reciprocal (v_rcp_f32, 1 ulp)
+// instead of an IEEE division: a division costs ten instructions, and the
+// network evaluates 384 of them per node.
+__device__ __forceinline__ float logistic(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+
+__device__ __forceinline__ float silu(float z) { return z * logistic(z); }
 
 __device__ __forceinline__ float dsilu(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = logistic(z);
   return s * (1.0f + z * (1.0f - s));
 }
 
