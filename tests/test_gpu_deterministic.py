@@ -1,9 +1,12 @@
 """Deterministic mode (otfem_set_deterministic): every per-node sum of the step
-is added in slot order, so the same step from the same state gives the same
-bits on every run -- single and batched meshes, plasticity, the learned start,
-every CG block size and record form, and across a checkpoint.  Its results
-meet the same 1e-6 per-step bar against the FP64 oracle, and stay within
-rounding of the default (LDS-atomic) mode."""
+has an order-independent result (K'p in CG as 64-bit fixed point with a
+per-tile scale from a stiffness bound, the once-per-step sums in slot order),
+so the same step from the same state gives the same bits on every run --
+single and batched meshes, plasticity, the learned start, every CG block size
+and record form, and across a checkpoint.  Its results meet the same 1e-6
+per-step bar against the FP64 oracle, stay within rounding of the default
+(LDS-atomic) mode also when the stiffness spans six decades inside one tile,
+and a non-finite input is still reported as such."""
 import numpy as np
 import pytest
 
@@ -176,3 +179,38 @@ def test_switching_off_returns_to_the_default_mode():
     x, _ = s.state()
     assert np.isfinite(x).all() and np.abs(x - c).max() > 0
     s.close()
+
+
+def test_fixed_point_sums_over_a_wide_stiffness_range():
+    """E log-uniform over 1e3..1e9 per tet: one tile's scale is set by its
+    stiffest tets, so the soft tets' terms use the fewest bits; CG to rel-res
+    1e-10 ends within 1e-8 of the default mode, with the same bits twice."""
+    from opentissue_amd import fem
+    c, t, fx, f, _, kw = _cube(seed=4)
+    E = np.exp(np.random.default_rng(4).uniform(np.log(1e3), np.log(1e9), len(t)))
+    P = fem.config_params(tol=1e-10, cap=20000)
+    a = _run(c, t, fx, f, E, kw, 4, P)
+    b = _run(c, t, fx, f, E, kw, 4, P)
+    d = _run(c, t, fx, f, E, kw, 4, P, deterministic=False)
+    err = rel_disp_error(a[0], d[0], c)
+    print("wide stiffness range: deterministic vs default", err, flush=True)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert err <= 1e-8
+
+
+def test_nonfinite_force_is_reported_in_deterministic_mode():
+    from opentissue_amd import fem
+    c, t, fx, f, E, kw = _cube()
+    s = fem.Solver(c, t, fx)
+    s.init_materials(E, 0.3, 1000.0)
+    s.set_deterministic(True)
+    bad = f.copy()
+    bad[np.nonzero(fx == 0)[0][0], 1] = np.inf
+    s.set_external_forces(bad)
+    with pytest.raises(fem.OtfemError) as e:
+        s.simulate(0.005, True, fem.config_params())
+    assert e.value.status == fem.NONFINITE
+    s.set_state(c, np.zeros_like(c))
+    s.set_external_forces(f)
+    st = s.simulate(0.005, True, fem.config_params())
+    assert st.flags == 0 and st.bodies_converged == 1
