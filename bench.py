@@ -100,6 +100,29 @@ def copy_probe():
                       "launches": 5}))
 
 
+def stream_rate():
+    """A streaming reference on this GPU: c = a + b over 2 GiB arrays (two
+    reads and one write per element), HIP events over 10 launches, GB/s of
+    bytes moved.  The rate a bandwidth-bound kernel can expect here."""
+    import torch
+    n = 1 << 29  # 2 GiB of float32 per array
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.ones_like(a)
+    c = torch.empty_like(a)
+    torch.add(a, b, out=c)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        torch.add(a, b, out=c)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) / 1e3 / 10
+    del a, b, c
+    torch.cuda.empty_cache()
+    return 3 * 4 * n / sec / 1e9
+
+
 # ---------------------------------------------------------------------------
 # CPU baselines (the FP64 oracle, one thread)
 
@@ -620,6 +643,14 @@ def main():
                     "(inputs and activations as BF16 hi + lo pairs, 3-output layer padded to 32 rows)"}
         result["training"] = train_info
         result["cantilever_ab"] = ab
+    if not args.profile:
+        sr = stream_rate()
+        result["stream_reference"] = {
+            "gbs": sr, "frac_of_peak": sr / HBM_PEAK_GBS,
+            "kernel": "torch add c = a + b, 2 GiB float32 arrays (2 reads + 1 write per element)",
+            "note": "measured in this run; the counted HBM rate of k_cg_iter (profiles/) sits "
+                    "against this, its algorithmic rate against the 8 TB/s peak"}
+        log(f"streaming reference (c = a + b, 2 GiB arrays): {sr:.0f} GB/s")
     if not args.no_episode and "learned" in legs:
         result["episode"] = episode(s, w, args, models)
         result["start_residuals"] = start_residuals(s, w, args, models)
