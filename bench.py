@@ -40,6 +40,7 @@ import argparse
 import json
 import os
 import platform
+import subprocess
 import sys
 import time
 
@@ -102,8 +103,21 @@ def copy_probe():
 
 def stream_rate():
     """A streaming reference on this GPU: c = a + b over 2 GiB arrays (two
-    reads and one write per element), HIP events over 10 launches, GB/s of
-    bytes moved.  The rate a bandwidth-bound kernel can expect here."""
+    reads and one write per element), GB/s of bytes moved: the rate a
+    bandwidth-bound kernel can expect here.  Measured in a child process
+    (torch brings its own HIP runtime, which does not start in a process
+    where the library's runtime already holds the GPU); None if it fails."""
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--stream-probe"],
+                       capture_output=True, text=True, timeout=300)
+    try:
+        return float(json.loads(r.stdout.strip().splitlines()[-1])["stream_gbs"])
+    except (ValueError, IndexError, KeyError):
+        log(f"streaming reference failed: {r.stderr.strip()[-300:]}")
+        return None
+
+
+def stream_probe():
+    """The child of stream_rate: HIP events over 10 launches of c = a + b."""
     import torch
     n = 1 << 29  # 2 GiB of float32 per array
     a = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -118,9 +132,7 @@ def stream_rate():
     e1.record()
     torch.cuda.synchronize()
     sec = e0.elapsed_time(e1) / 1e3 / 10
-    del a, b, c
-    torch.cuda.empty_cache()
-    return 3 * 4 * n / sec / 1e9
+    print(json.dumps({"stream_gbs": 3 * 4 * n / sec / 1e9}))
 
 
 # ---------------------------------------------------------------------------
@@ -538,6 +550,7 @@ def main():
                     help="owned nodes per tile (0: the default tiling; -1: the layout autotuner "
                          "picks the tile target and the CG block size)")
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--stream-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-learned", action="store_true", help="plain leg only")
     ap.add_argument("--legs", default="learned,plain,subspace,network",
                     help="legs to time (learned: network + subspace start; subspace: without the "
@@ -566,6 +579,9 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
+        return
+    if args.stream_probe:
+        stream_probe()
         return
 
     legs = ("plain",) if args.no_learned else tuple(args.legs.split(","))
@@ -643,8 +659,8 @@ def main():
                     "(inputs and activations as BF16 hi + lo pairs, 3-output layer padded to 32 rows)"}
         result["training"] = train_info
         result["cantilever_ab"] = ab
-    if not args.profile:
-        sr = stream_rate()
+    sr = None if args.profile else stream_rate()
+    if sr:
         result["stream_reference"] = {
             "gbs": sr, "frac_of_peak": sr / HBM_PEAK_GBS,
             "kernel": "torch add c = a + b, 2 GiB float32 arrays (2 reads + 1 write per element)",

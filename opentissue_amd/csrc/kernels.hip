@@ -1280,50 +1280,41 @@ __global__ __launch_bounds__(kSubGramBlock) void k_sub_gram(StepArgs A) {
     valid = i < pe && live;
     load_raw<ST>(A, S, A.prim_slot[min(i, pe - 1)], raw);
   };
-  // this lane's direction at the slot's four corners, from LDS (lanes past
-  // the last direction read the last one; their rows are zeroed)
-  const int dsel = min(dir, k - 1);
-  auto gather = [&](const SlotRaw<ST> &raw, float g[4][3]) {
-    const int cn[4] = {raw.c.x, raw.c.y, raw.c.z, raw.c.w};
+  auto rows = [&](const SlotRaw<ST> &raw, bool valid) {
+    double e[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (valid) {
+      double bt[3][3], D[3], f[3];
+      unpack_raw<double, ST>(raw, Dt, bt, D);
+      if (std::is_same<ST, Folded>::value) {
+        f[0] = ft[0];
+        f[1] = ft[1];
+        f[2] = ft[2];
+      } else {
+        sub_factors(D, cK, f);
+      }
+      const int cn[4] = {raw.c.x, raw.c.y, raw.c.z, raw.c.w};
+      const float *p0 = sdir + ((int64_t)cn[0] * k + dir) * 3;
+      double dd[3][3];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const float *pq = sdir + (cn[q] * k + dsel) * 3;
-      g[q][0] = pq[0];
-      g[q][1] = pq[1];
-      g[q][2] = pq[2];
+      for (int kk = 0; kk < 3; ++kk) {
+        const float *pk = sdir + ((int64_t)cn[kk + 1] * k + dir) * 3;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) dd[kk][x] = (double)pk[x] - (double)p0[x];
+      }
+      double G[9];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          G[3 * r + c] = dd[0][r] * bt[0][c] + dd[1][r] * bt[1][c] + dd[2][r] * bt[2][c];
+      const double e0 = G[0], e1 = G[4], e2 = G[8];
+      e[0] = f[0] * (e0 + e1 + e2);
+      e[1] = f[1] * (e0 - e1) * 0.70710678118654752440;
+      e[2] = f[1] * (e0 + e1 - 2.0 * e2) * 0.40824829046386301637;
+      e[3] = f[2] * (G[1] + G[3]);
+      e[4] = f[2] * (G[2] + G[6]);
+      e[5] = f[2] * (G[5] + G[7]);
     }
-  };
-  auto rows = [&](const SlotRaw<ST> &raw, bool valid, const float g[4][3]) {
-    double bt[3][3], D[3], f[3];
-    unpack_raw<double, ST>(raw, Dt, bt, D);
-    if (std::is_same<ST, Folded>::value) {
-      f[0] = ft[0];
-      f[1] = ft[1];
-      f[2] = ft[2];
-    } else {
-      sub_factors(D, cK, f);
-    }
-    double dd[3][3];
-#pragma unroll
-    for (int kk = 0; kk < 3; ++kk)
-#pragma unroll
-      for (int x = 0; x < 3; ++x) dd[kk][x] = (double)g[kk + 1][x] - (double)g[0][x];
-    double G[9];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-        G[3 * r + c] = dd[0][r] * bt[0][c] + dd[1][r] * bt[1][c] + dd[2][r] * bt[2][c];
-    const double e0 = G[0], e1 = G[4], e2 = G[8];
-    double e[6];
-    e[0] = f[0] * (e0 + e1 + e2);
-    e[1] = f[1] * (e0 - e1) * 0.70710678118654752440;
-    e[2] = f[1] * (e0 + e1 - 2.0 * e2) * 0.40824829046386301637;
-    e[3] = f[2] * (G[1] + G[3]);
-    e[4] = f[2] * (G[2] + G[6]);
-    e[5] = f[2] * (G[5] + G[7]);
-#pragma unroll
-    for (int q = 0; q < 6; ++q) e[q] = valid ? e[q] : 0.0;
     // two accumulators: consecutive MFMAs do not wait for each other
     C0 = __builtin_amdgcn_mfma_f64_16x16x4f64(e[0], e[0], C0, 0, 0, 0);
     C1 = __builtin_amdgcn_mfma_f64_16x16x4f64(e[1], e[1], C1, 0, 0, 0);
@@ -1336,17 +1327,6 @@ __global__ __launch_bounds__(kSubGramBlock) void k_sub_gram(StepArgs A) {
   // the previous U are used (wave-uniform loop; the larger unfolded records
   // keep 2 in flight, within the registers of 3 waves per SIMD)
   constexpr int U = std::is_same<ST, Folded>::value ? kSubRounds : 2;
-  // the U rounds of one buffer, each round's LDS reads issued while the
-  // previous round computes
-  auto rounds = [&](const SlotRaw<ST> (&rr)[U], const bool (&vv)[U]) {
-    float g[2][4][3];
-    gather(rr[0], g[0]);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (u + 1 < U) gather(rr[u + 1], g[(u + 1) & 1]);
-      rows(rr[u], vv[u], g[u & 1]);
-    }
-  };
   constexpr int R = 4 * W;  // tets per round over the workgroup
   int base = pb + 4 * wave + qs;
   if (base - qs < pe) {
@@ -1357,11 +1337,13 @@ __global__ __launch_bounds__(kSubGramBlock) void k_sub_gram(StepArgs A) {
     for (;; base += 2 * U * R) {
 #pragma unroll
       for (int u = 0; u < U; ++u) fetch(base + (U + u) * R, rb[u], vb[u]);
-      rounds(ra, va);
+#pragma unroll
+      for (int u = 0; u < U; ++u) rows(ra[u], va[u]);
       if (base - qs + U * R >= pe) break;
 #pragma unroll
       for (int u = 0; u < U; ++u) fetch(base + (2 * U + u) * R, ra[u], va[u]);
-      rounds(rb, vb);
+#pragma unroll
+      for (int u = 0; u < U; ++u) rows(rb[u], vb[u]);
       if (base - qs + 2 * U * R >= pe) break;
     }
   }
@@ -1374,7 +1356,7 @@ __global__ __launch_bounds__(kSubGramBlock) void k_sub_gram(StepArgs A) {
       const int64_t g = ti.node_begin + l;
       if (!A.fixed[g]) {
         const double sq = sqrt(cM * A.mass[g]);
-        const float *d = sdir + (l * k + dir) * 3;
+        const float *d = sdir + ((int64_t)l * k + dir) * 3;
 #pragma unroll
         for (int x = 0; x < 3; ++x) {
           e[x] = sq * (double)d[x];
