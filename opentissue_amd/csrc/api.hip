@@ -770,6 +770,10 @@ StepArgs make_args(otfem_solver *h, double dt, int warp, const otfem_params &P) 
   a.tile_d = h->tile_d;
   a.tile_kb = h->tile_kb;
   a.folded = (h->one_nu_per_tile && h->lc.spmv_mode == SPMV_F64) ? 1 : 0;
+  // FP32 directions where CG runs to a relative residual: the converged
+  // iterate does not depend on them beyond the tolerance, while the
+  // fixed-count iterate of the reference mode would
+  a.p32 = (P.tol_mode == 1 && h->lc.spmv_mode == SPMV_F64) ? 1 : 0;
   a.ordered = h->ordered ? 1 : 0;
   a.ord_begin = h->ord_begin;
   a.ord_ent = h->ord_ent;

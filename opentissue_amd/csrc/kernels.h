@@ -35,7 +35,8 @@ struct StepArgs {
   const uint8_t *fixed;
   double *rhs;
   double *r;             // [2][3N] residual, alternating by CG iteration
-  double *p;             // [2][3N] search direction
+  double *p;             // [2][3N] search direction (FP64), or [2][3N] FP32 in its first half
+                         //         with `p32`
   double *y;             // [2][3N] K' p
   // tiles / slots
   const TileInfo *tiles;
@@ -87,6 +88,9 @@ struct StepArgs {
   int32_t plastic_on;
   int32_t store64;       // element kernel also writes bt8
   int32_t folded;        // slot records in folded form (every tile has one Poisson ratio)
+  int32_t p32;           // the CG direction p is stored in FP32 (CG to a relative residual in
+                         // the default SpMV mode): the SpMV, the dot products and the updates
+                         // of v and r all use the stored value, so r stays the residual of v
   // Ordered sums (deterministic mode): slots store their per-corner terms in
   // `ordered_terms` (slot-major: one contiguous record per slot) and every
   // owned node adds its terms in slot order, so results do not depend on

@@ -787,22 +787,25 @@ struct CgShape {
   static constexpr bool kEarlyPartials = BLOCK <= 256;
 };
 
-// One tile node's inputs to the CG prologue, as loaded.
+// One tile node's inputs to the CG prologue, as loaded (PT: the storage
+// type of the direction p).
+template <typename PT>
 struct NodeIn {
   int32_t g;  // internal node id
   int32_t fx;
   double m;
-  double r[3], p[3], y[3], v[3];
+  double r[3], y[3], v[3];
+  PT p[3];
 };
 
 // Loads the inputs of tile nodes l = base + threadIdx.x + k*BLOCK (k < NB)
 // without using any of them, so every load of the batch is in flight at
 // once.  Lanes past the node table read node_begin (in bounds) and are never
 // used.
-template <int BLOCK, int NB>
+template <int BLOCK, int NB, typename PT>
 __device__ __forceinline__ void load_nodes(const StepArgs &A, const TileInfo &ti, int it,
-                                           int base, const double *Rp, const double *Pp,
-                                           const double *Yp, NodeIn (&nd)[NB]) {
+                                           int base, const double *Rp, const PT *Pp,
+                                           const double *Yp, NodeIn<PT> (&nd)[NB]) {
   const int nn = ti.n_owned + ti.n_halo, no = ti.n_owned;
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
@@ -838,11 +841,13 @@ __device__ __forceinline__ void load_nodes(const StepArgs &A, const TileInfo &ti
 // r = r - alpha q, p = r + beta p (q = cM m p + cK y), v += alpha p for owned
 // nodes; p goes to LDS for the slot loop, r and p of owned nodes to LDS and
 // memory, and the mass of owned nodes to LDS (-1 marks a fixed node).  Fixed
-// nodes carry zero r and p.
-template <typename AT>
+// nodes carry zero r and p.  p is rounded to its storage type PT before any
+// use: the SpMV, the dot products and (next kernel) v += alpha p and
+// r -= alpha A p all see the same vector, so r stays the residual of v.
+template <typename AT, typename PT>
 __device__ __forceinline__ void update_node(const StepArgs &A, int it, bool stop, double alpha,
-                                            double beta, int l, int no, const NodeIn &n,
-                                            double *Rn, double *Pn, double *sp, double *sr,
+                                            double beta, int l, int no, const NodeIn<PT> &n,
+                                            double *Rn, PT *Pn, double *sp, double *sr,
                                             AT *sy, double *sm, double &pmax) {
   const int64_t g = n.g;
   double rn[3], pn[3];
@@ -864,6 +869,12 @@ __device__ __forceinline__ void update_node(const StepArgs &A, int it, bool stop
       for (int d = 0; d < 3; ++d) A.v[3 * g + d] = n.v[d] + alpha * n.p[d];
     }
   }
+  PT pf[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    pf[d] = (PT)pn[d];
+    pn[d] = (double)pf[d];
+  }
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     sp[3 * l + d] = pn[d];
@@ -874,7 +885,7 @@ __device__ __forceinline__ void update_node(const StepArgs &A, int it, bool stop
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       if (it > 0 && !stop) Rn[3 * g + d] = rn[d];
-      Pn[3 * g + d] = pn[d];
+      Pn[3 * g + d] = pf[d];
       sr[3 * l + d] = rn[d];
       sy[3 * l + d] = (AT)0;
     }
@@ -916,7 +927,7 @@ __device__ __forceinline__ void node_partials(const StepArgs &A, int64_t g, int 
 // before the body's partials are summed, so the kernel waits for memory about
 // three times (tile record, node ids, everything else) rather than once per
 // node loop trip.  The slot loop is double-buffered.
-template <int BLOCK, typename MT, typename AT, typename ST>
+template <int BLOCK, typename MT, typename AT, typename ST, typename PT>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(CgShape<BLOCK>::kWaves)))
 void k_cg_iter(StepArgs A, int it) {
   extern __shared__ double lds[];
@@ -955,8 +966,10 @@ void k_cg_iter(StepArgs A, int it) {
   const int nt = A.ntiles;
   const int nn = ti.n_owned + ti.n_halo, no = ti.n_owned;
   const int cur = it & 1, prv = (it - 1) & 1;
-  double *Rn = A.r + cur * 3 * A.N, *Pn = A.p + cur * 3 * A.N, *Yn = A.y + cur * 3 * A.N;
-  const double *Rp = A.r + prv * 3 * A.N, *Pp = A.p + prv * 3 * A.N, *Yp = A.y + prv * 3 * A.N;
+  double *Rn = A.r + cur * 3 * A.N, *Yn = A.y + cur * 3 * A.N;
+  PT *Pn = reinterpret_cast<PT *>(A.p) + cur * 3 * A.N;
+  const double *Rp = A.r + prv * 3 * A.N, *Yp = A.y + prv * 3 * A.N;
+  const PT *Pp = reinterpret_cast<const PT *>(A.p) + prv * 3 * A.N;
   // tile partials: r.r and b.b of the element kernel at it == 0, else
   // p.q, r.q, q.q, r.r of the previous kernel
   const double4 *prevp = reinterpret_cast<const double4 *>(A.part4 + prv * 4 * (int64_t)nt);
@@ -977,8 +990,8 @@ void k_cg_iter(StepArgs A, int it) {
     for (int u = 0; u < U; ++u)
       load_raw<ST>(A, S, min(sb + (int)threadIdx.x + u * BLOCK, send - 1), ra[u]);
   }
-  NodeIn nd[NB];
-  load_nodes<BLOCK, NB>(A, ti, it, 0, Rp, Pp, Yp, nd);
+  NodeIn<PT> nd[NB];
+  load_nodes<BLOCK, NB, PT>(A, ti, it, 0, Rp, Pp, Yp, nd);
 
   // exit if the body stopped in an earlier kernel (a stop recorded by this
   // kernel still needs this workgroup's prologue: v += alpha p)
@@ -1042,11 +1055,11 @@ void k_cg_iter(StepArgs A, int it) {
     for (int k = 0; k < NB; ++k) {
       const int l = base + (int)threadIdx.x + k * BLOCK;
       if (l < nn)
-        update_node<Acc>(A, it, stop, alpha, beta, l, no, nd[k], Rn, Pn, sp, sr, sy, sm, pmax);
+        update_node<Acc, PT>(A, it, stop, alpha, beta, l, no, nd[k], Rn, Pn, sp, sr, sy, sm, pmax);
     }
     base += NB * BLOCK;
     if (base >= nn) break;
-    load_nodes<BLOCK, NB>(A, ti, it, base, Rp, Pp, Yp, nd);
+    load_nodes<BLOCK, NB, PT>(A, ti, it, base, Rp, Pp, Yp, nd);
   }
   if (stop) return;
   if (kFixed && pmax > 0.0) atomicMax(&s_pmax, (unsigned long long)__double_as_longlong(pmax));
@@ -1658,12 +1671,15 @@ __global__ __launch_bounds__(BLOCK) void k_finish(StepArgs A) {
     const double alpha = rrd / pq;
     const bool ok = finite(alpha);
     if (ok) {
-      const double *Pp = A.p + ((it - 1) & 1) * 3 * A.N;
+      const int64_t off = ((it - 1) & 1) * 3 * A.N;
+      const double *Pp = A.p + off;
+      const float *Pf = reinterpret_cast<const float *>(A.p) + off;
       for (int l = threadIdx.x; l < ti.n_owned; l += BLOCK) {
         const int64_t g = ti.node_begin + l;
         if (A.fixed[g]) continue;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) A.v[3 * g + d] += alpha * Pp[3 * g + d];
+        for (int d = 0; d < 3; ++d)
+          A.v[3 * g + d] += alpha * (A.p32 ? (double)Pf[3 * g + d] : Pp[3 * g + d]);
       }
     }
     if (ti.body_rank == 0 && threadIdx.x == 0) {
@@ -1706,12 +1722,19 @@ size_t cg_lds_bytes(int max_nodes, int max_owned) {
 
 namespace {
 
-template <int BLOCK, typename MT, typename AT, typename ST>
+template <int BLOCK, typename MT, typename AT, typename ST, typename PT = double>
 hipError_t enqueue_iterations(const StepArgs &a, size_t lds_cg, hipStream_t s, int it0, int it1,
                               int grid) {
   for (int it = it0; it < it1; ++it)
-    hipLaunchKernelGGL((k_cg_iter<BLOCK, MT, AT, ST>), dim3(grid), dim3(BLOCK), lds_cg, s, a, it);
+    hipLaunchKernelGGL((k_cg_iter<BLOCK, MT, AT, ST, PT>), dim3(grid), dim3(BLOCK), lds_cg, s, a, it);
   return hipGetLastError();
+}
+
+// FP32 directions (a.p32) or FP64 ones.
+template <int BLOCK, typename MT, typename AT, typename ST>
+hipError_t enqueue_p(const StepArgs &a, size_t lds_cg, hipStream_t s, int it0, int it1, int grid) {
+  if (a.p32) return enqueue_iterations<BLOCK, MT, AT, ST, float>(a, lds_cg, s, it0, it1, grid);
+  return enqueue_iterations<BLOCK, MT, AT, ST, double>(a, lds_cg, s, it0, it1, grid);
 }
 
 // Active-tile list: tiles whose body has not stopped, in tile order, and
@@ -1751,17 +1774,16 @@ hipError_t enqueue_mode(const StepArgs &a, int mode, size_t lds_cg, hipStream_t 
   if (a.ordered) {  // deterministic mode: FP64 arithmetic, fixed-point node sums
     if (mode == SPMV_F64_STORE64)
       return enqueue_iterations<BLOCK, double, Fixed, double>(a, lds_cg, s, it0, it1, g);
-    if (a.folded) return enqueue_iterations<BLOCK, double, Fixed, Folded>(a, lds_cg, s, it0, it1, g);
-    return enqueue_iterations<BLOCK, double, Fixed, float>(a, lds_cg, s, it0, it1, g);
+    if (a.folded) return enqueue_p<BLOCK, double, Fixed, Folded>(a, lds_cg, s, it0, it1, g);
+    return enqueue_p<BLOCK, double, Fixed, float>(a, lds_cg, s, it0, it1, g);
   }
   switch (mode) {
     case SPMV_F32: return enqueue_iterations<BLOCK, float, float, float>(a, lds_cg, s, it0, it1, g);
     case SPMV_F32_ACC64:
       return enqueue_iterations<BLOCK, float, double, float>(a, lds_cg, s, it0, it1, g);
     case SPMV_F64:
-      if (a.folded)
-        return enqueue_iterations<BLOCK, double, double, Folded>(a, lds_cg, s, it0, it1, g);
-      return enqueue_iterations<BLOCK, double, double, float>(a, lds_cg, s, it0, it1, g);
+      if (a.folded) return enqueue_p<BLOCK, double, double, Folded>(a, lds_cg, s, it0, it1, g);
+      return enqueue_p<BLOCK, double, double, float>(a, lds_cg, s, it0, it1, g);
     case SPMV_F64_STORE64:
       return enqueue_iterations<BLOCK, double, double, double>(a, lds_cg, s, it0, it1, g);
     default: return hipErrorInvalidValue;

@@ -254,8 +254,10 @@ def test_batched_bodies_match_per_body_oracle(mode):
             err = rel_disp_error(xg[sl], x1, parts[b][0])
             assert err <= TOL, (step, b, err)
             # the iteration that crosses rel-res 1e-6 moves with rounding order
-            # (LDS-atomic accumulation), by a few iterations at most
-            slack = max(3, int(0.05 * st.iterations))
+            # (LDS-atomic accumulation) and, in this mode, with the FP32
+            # storage of the CG direction: by a few iterations (measured: up
+            # to 5 of 50 on the smallest bodies, +1% on average)
+            slack = max(3, int(0.12 * st.iterations)) if mode == "cfg" else 0
             assert abs(int(its_g[b]) - st.iterations) <= slack, (step, b, its_g[b], st.iterations)
 
 
