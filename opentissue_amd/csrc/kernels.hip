@@ -649,17 +649,17 @@ __device__ __forceinline__ void unpack_raw<double, Folded>(const SlotRaw<Folded>
 
 // (K' p)_e for one slot, per corner: G = sum_k (p_k - p_0) bt_k^T,
 // sigma = D(G), f_k = sigma bt_k, f_0 = -(f_1 + f_2 + f_3).
-template <typename MT>
+template <typename MT, typename PS>
 __device__ __forceinline__ void slot_forces(ushort4 c, const MT bt[3][3], const MT D[3],
-                                            const double *sp, MT f[4][3]) {
+                                            const PS *sp, MT f[4][3]) {
   const double p0x = sp[3 * c.x], p0y = sp[3 * c.x + 1], p0z = sp[3 * c.x + 2];
   MT dp[3][3];
   const int cn[3] = {c.y, c.z, c.w};
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    dp[k][0] = (MT)(sp[3 * cn[k]] - p0x);
-    dp[k][1] = (MT)(sp[3 * cn[k] + 1] - p0y);
-    dp[k][2] = (MT)(sp[3 * cn[k] + 2] - p0z);
+    dp[k][0] = (MT)((double)sp[3 * cn[k]] - p0x);
+    dp[k][1] = (MT)((double)sp[3 * cn[k] + 1] - p0y);
+    dp[k][2] = (MT)((double)sp[3 * cn[k] + 2] - p0z);
   }
   MT G[9];
 #pragma unroll
@@ -679,11 +679,11 @@ __device__ __forceinline__ void slot_forces(ushort4 c, const MT bt[3][3], const 
 }
 
 // y_owned += (K' p)_e for one slot (LDS atomics).
-template <typename MT, typename AT>
+template <typename MT, typename AT, typename PS>
 __device__ __forceinline__ void slot_spmv(ushort4 c, const MT bt[3][3], const MT D[3],
-                                          const double *sp, AT *sy, int no) {
+                                          const PS *sp, AT *sy, int no) {
   MT f[4][3];
-  slot_forces<MT>(c, bt, D, sp, f);
+  slot_forces<MT, PS>(c, bt, D, sp, f);
   const int cc[4] = {c.x, c.y, c.z, c.w};
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -697,11 +697,11 @@ __device__ __forceinline__ void slot_spmv(ushort4 c, const MT bt[3][3], const MT
 // Deterministic mode: the slot's four corner terms to out[12] (one
 // contiguous record in memory); each owned node adds its terms in slot order
 // after the slot loop.
-template <typename MT>
+template <typename MT, typename PS>
 __device__ __forceinline__ void slot_spmv_store(ushort4 c, const MT bt[3][3], const MT D[3],
-                                                const double *sp, double *out) {
+                                                const PS *sp, double *out) {
   MT f[4][3];
-  slot_forces<MT>(c, bt, D, sp, f);
+  slot_forces<MT, PS>(c, bt, D, sp, f);
 #pragma unroll
   for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -753,12 +753,12 @@ __device__ __forceinline__ unsigned long long to_fixed(double x, double sc) {
 }
 
 // y_owned += (K' p)_e for one slot as fixed point (LDS integer atomics).
-template <typename MT>
+template <typename MT, typename PS>
 __device__ __forceinline__ void slot_spmv_fixed(ushort4 c, const MT bt[3][3], const MT D[3],
-                                                const double *sp, unsigned long long *sy, int no,
+                                                const PS *sp, unsigned long long *sy, int no,
                                                 double sc) {
   MT f[4][3];
-  slot_forces<MT>(c, bt, D, sp, f);
+  slot_forces<MT, PS>(c, bt, D, sp, f);
   const int cc[4] = {c.x, c.y, c.z, c.w};
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -847,7 +847,7 @@ __device__ __forceinline__ void load_nodes(const StepArgs &A, const TileInfo &ti
 template <typename AT, typename PT>
 __device__ __forceinline__ void update_node(const StepArgs &A, int it, bool stop, double alpha,
                                             double beta, int l, int no, const NodeIn<PT> &n,
-                                            double *Rn, PT *Pn, double *sp, double *sr,
+                                            double *Rn, PT *Pn, PT *sp, double *sr,
                                             AT *sy, double *sm, double &pmax) {
   const int64_t g = n.g;
   double rn[3], pn[3];
@@ -877,7 +877,7 @@ __device__ __forceinline__ void update_node(const StepArgs &A, int it, bool stop
   }
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    sp[3 * l + d] = pn[d];
+    sp[3 * l + d] = pf[d];
     // a non-finite p makes the bound infinite (max ignores NaN)
     pmax = isfinite(pn[d]) ? fmax(pmax, fabs(pn[d])) : HUGE_VAL;
   }
@@ -895,9 +895,9 @@ __device__ __forceinline__ void update_node(const StepArgs &A, int it, bool stop
 
 // Owned node l's contribution to the tile partials p.q, r.q, q.q, r.r, and
 // its K'p to memory.
-template <typename AT>
+template <typename AT, typename PS>
 __device__ __forceinline__ void node_partials(const StepArgs &A, int64_t g, int l, double m,
-                                              const double *sp, const double *sr, const AT *sy,
+                                              const PS *sp, const double *sr, const AT *sy,
                                               double *Yn, double4 &acc, double inv) {
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -1042,10 +1042,11 @@ void k_cg_iter(StepArgs A, int it) {
       bs->iterations = it;
   }
 
-  double *sp = lds;                 // p of every tile node
-  double *sr = sp + 3 * nn;         // r of owned nodes
+  double *sr = lds;                 // r of owned nodes
   double *sm = sr + 3 * no;         // mass of owned nodes, -1 if fixed
   Acc *sy = reinterpret_cast<Acc *>(sm + no);  // K'p of owned nodes
+  // p of every tile node, in its storage type (after the 8-byte arrays)
+  PT *sp = reinterpret_cast<PT *>(sm + no + (3 * no * sizeof(Acc) + 7) / 8);
 
   // prologue: finish iteration it-1 and form the new direction, NB nodes per
   // thread at a time (the first batch was loaded above)
@@ -1090,11 +1091,11 @@ void k_cg_iter(StepArgs A, int it) {
           MT bt[3][3], D[3];
           unpack_raw<MT, ST>(ra[u], Dt, bt, D);
           if constexpr (kOrdered)
-            slot_spmv_store<MT>(ra[u].c, bt, D, sp, A.ordered_terms + 12 * (int64_t)(s + u * BLOCK));
+            slot_spmv_store<MT, PT>(ra[u].c, bt, D, sp, A.ordered_terms + 12 * (int64_t)(s + u * BLOCK));
           else if constexpr (kFixed)
-            slot_spmv_fixed<MT>(ra[u].c, bt, D, sp, sy, no, sc);
+            slot_spmv_fixed<MT, PT>(ra[u].c, bt, D, sp, sy, no, sc);
           else
-            slot_spmv<MT, AT>(ra[u].c, bt, D, sp, sy, no);
+            slot_spmv<MT, AT, PT>(ra[u].c, bt, D, sp, sy, no);
         }
       }
       if (s + U * BLOCK >= send) break;
@@ -1107,12 +1108,12 @@ void k_cg_iter(StepArgs A, int it) {
           MT bt[3][3], D[3];
           unpack_raw<MT, ST>(rb[u], Dt, bt, D);
           if constexpr (kOrdered)
-            slot_spmv_store<MT>(rb[u].c, bt, D, sp,
+            slot_spmv_store<MT, PT>(rb[u].c, bt, D, sp,
                                 A.ordered_terms + 12 * (int64_t)(s + (U + u) * BLOCK));
           else if constexpr (kFixed)
-            slot_spmv_fixed<MT>(rb[u].c, bt, D, sp, sy, no, sc);
+            slot_spmv_fixed<MT, PT>(rb[u].c, bt, D, sp, sy, no, sc);
           else
-            slot_spmv<MT, AT>(rb[u].c, bt, D, sp, sy, no);
+            slot_spmv<MT, AT, PT>(rb[u].c, bt, D, sp, sy, no);
         }
       }
     }
@@ -1138,7 +1139,7 @@ void k_cg_iter(StepArgs A, int it) {
       sy[3 * l + 1] = y1;
       sy[3 * l + 2] = y2;
     }
-    if (m >= 0.0) node_partials<Acc>(A, ti.node_begin + l, l, m, sp, sr, sy, Yn, acc, inv);
+    if (m >= 0.0) node_partials<Acc, PT>(A, ti.node_begin + l, l, m, sp, sr, sy, Yn, acc, inv);
   }
   const double4 tot = block_sum4<BLOCK>(acc, red);
   if (threadIdx.x == 0) reinterpret_cast<double4 *>(A.part4 + cur * 4 * (int64_t)nt)[tile] = tot;
@@ -1571,9 +1572,9 @@ __global__ __launch_bounds__(BLOCK) void k_sub_start(StepArgs A) {
     double bt[3][3], D[3];
     unpack_raw<double, ST>(raw, Dt, bt, D);
     if constexpr (kOrdered)
-      slot_spmv_store<double>(raw.c, bt, D, sp, A.ordered_terms + 12 * (int64_t)s);
+      slot_spmv_store<double, double>(raw.c, bt, D, sp, A.ordered_terms + 12 * (int64_t)s);
     else
-      slot_spmv<double, double>(raw.c, bt, D, sp, sy, no);
+      slot_spmv<double, double, double>(raw.c, bt, D, sp, sy, no);
   }
   __syncthreads();
   double rr = 0.0;
