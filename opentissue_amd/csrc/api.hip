@@ -81,7 +81,7 @@ struct otfem_solver {
   int *done_count = nullptr;  // [2]: stopped bodies, active tiles
   int32_t *act = nullptr;     // [ntiles] active-tile list of batched meshes
   unsigned long long *iter_acc = nullptr, *unconv_acc = nullptr;
-  size_t lds_elem = 0, lds_cg = 0;
+  size_t lds_elem = 0, lds_cg = 0, lds_cg32 = 0;  // lds_cg32: CG kernels with FP32 directions
   LaunchConfig lc;
   // captured step graphs (element part, CG part) and the inputs they bake in
   bool use_graphs = true;
@@ -727,6 +727,7 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
 
   h->lds_elem = lds_elem;
   h->lds_cg = lds_cg;
+  h->lds_cg32 = cg_lds_bytes(L.max_tile_nodes, L.max_tile_owned, 4);
   // material data changed: captured graphs are rebuilt on the next step
   drop_graphs(h);
   h->initialized = true;
@@ -1284,6 +1285,7 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
   // from it covers the list the next chunk builds; grids are powers of two
   // times 256, capped at the tile count.
   const int ntiles = (int)h->L.tiles.size();
+  const size_t lds_cg = a.p32 ? h->lds_cg32 : h->lds_cg;
   int known_active = ntiles;
   const int tail_from = h->cg_stop_seen > 0 ? std::max(0, h->cg_stop_seen - 16) : cap;
   int prev_end = 0;
@@ -1300,13 +1302,13 @@ otfem_status otfem_step(otfem_handle h, double dt, int use_stiffness_warping,
       hipGraphExec_t &ge = h->g_chunks[std::make_tuple(it0, it1 - it0, grid)];
       if (!ge) {
         otfem_status cs = capture(
-            [&](hipStream_t cs_) { return enqueue_cg(a, h->lc, h->lds_cg, cs_, it0, it1, grid); },
+            [&](hipStream_t cs_) { return enqueue_cg(a, h->lc, lds_cg, cs_, it0, it1, grid); },
             &ge, "enqueue_cg");
         if (cs != OTFEM_OK) return cs;
       }
       HIPCHK(h, hipGraphLaunch(ge, s));
     } else {
-      HIPCHK(h, enqueue_cg(a, h->lc, h->lds_cg, s, it0, it1, grid));
+      HIPCHK(h, enqueue_cg(a, h->lc, lds_cg, s, it0, it1, grid));
     }
     if (it1 < cap) {
       HIPCHK(h, hipMemcpyAsync(&h->done_host[2 * (j & 1)], h->done_count, 2 * sizeof(int),
@@ -1480,7 +1482,7 @@ otfem_status otfem_step_noalloc(otfem_handle h, double dt, int use_stiffness_war
       while (grid < known_active) grid *= 2;
       grid = std::min(grid, nt);
     }
-    HIPCHK(h, enqueue_cg(a, h->lc, h->lds_cg, s, it0, it1, grid));
+    HIPCHK(h, enqueue_cg(a, h->lc, a.p32 ? h->lds_cg32 : h->lds_cg, s, it0, it1, grid));
     if (it1 < cap) {
       HIPCHK(h, hipMemcpyAsync(&h->done_host[2 * (j & 1)], a.done_count, 2 * sizeof(int),
                                hipMemcpyDeviceToHost, s));

@@ -158,6 +158,8 @@ hipError_t enqueue_surface_update(int64_t n, const int4 *ids, const double4 *w, 
 hipError_t enqueue_surface_normals(int64_t n, const int32_t *tet, const double *Re, int use_re,
                                    const double *n0, double *out, hipStream_t s);
 size_t element_lds_bytes(int max_nodes, int max_owned, int plastic_on);
-size_t cg_lds_bytes(int max_nodes, int max_owned);
+// LDS of the CG kernels (and of k_sub_start with p_bytes = 8); p_bytes is the
+// size of the direction's storage type (8, or 4 with p32).
+size_t cg_lds_bytes(int max_nodes, int max_owned, int p_bytes = 8);
 
 }  // namespace otfem

@@ -775,12 +775,14 @@ __device__ __forceinline__ void slot_spmv_fixed(ushort4 c, const MT bt[3][3], co
 // thread whose loads are all issued before any is used (one batch), slot
 // records per buffer of the double-buffered slot loop (FP64 records take
 // twice the registers), and the waves per SIMD the register allocation must
-// allow.  512 threads at 4 waves per SIMD run two tiles per CU.
+// allow.  512 threads at 4 waves per SIMD run two tiles per CU; 256 threads
+// at 3 waves per SIMD three (2 node batches: 4 batches at 2 waves measured
+// 2.6% slower on the 2,048-body set).
 template <int BLOCK>
 struct CgShape {
-  static constexpr int kNodes = BLOCK <= 256 ? 4 : (BLOCK <= 512 ? 2 : 1);
+  static constexpr int kNodes = BLOCK <= 512 ? 2 : 1;
   static constexpr int kSlots = BLOCK <= 256 ? 2 : 1;
-  static constexpr int kWaves = BLOCK <= 256 ? 2 : 4;
+  static constexpr int kWaves = BLOCK <= 256 ? 3 : 4;
   // the stop test waits for nothing else and the body's tile partials are
   // loaded with the other early loads (with 128 registers per thread: stop
   // test first, partials after the node inputs)
@@ -1715,10 +1717,10 @@ size_t element_lds_bytes(int max_nodes, int max_owned, int plastic_on) {
   return sizeof(double) * (6 * (size_t)max_nodes + (plastic_on ? 9 : 6) * (size_t)max_owned);
 }
 
-size_t cg_lds_bytes(int max_nodes, int max_owned) {
-  // FP64 p of all tile nodes; FP64 r, mass and K'p accumulators of owned
-  // nodes (FP64 covers every mode)
-  return sizeof(double) * 3 * (size_t)max_nodes + sizeof(double) * 7 * (size_t)max_owned;
+size_t cg_lds_bytes(int max_nodes, int max_owned, int p_bytes) {
+  // p of all tile nodes in its storage type; FP64 r and mass and the K'p
+  // accumulators (at most 8 bytes) of owned nodes
+  return (size_t)p_bytes * 3 * (size_t)max_nodes + sizeof(double) * 7 * (size_t)max_owned;
 }
 
 namespace {
