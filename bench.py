@@ -410,15 +410,17 @@ def roofline(w, leg, profile_key):
     """HBM roofline of k_cg_iter from a leg's own HIP events and iteration
     counts; counter figures from the committed rocprofv3 summaries of this
     workload, when there are any (marked as such)."""
-    traffic = profile_avg_us = None
+    traffic = profile_avg_us = traffic_build = profile_build = None
     tpath = os.path.join(ROOT, "profiles", f"k_cg_iter_{profile_key}_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        traffic, traffic_build = tj.get("hbm_bytes_per_launch"), tj.get("build")
     kpath = os.path.join(ROOT, "profiles", f"k_cg_iter_{profile_key}_kernel_trace.json")
     if os.path.exists(kpath):
         with open(kpath) as f:
-            profile_avg_us = json.load(f).get("avg_us_per_working_launch")
+            kj = json.load(f)
+        profile_avg_us, profile_build = kj.get("avg_us_per_working_launch"), kj.get("build")
     achieved = leg["cg_hbm_gbs"]
     out = {
         "bound": "hbm",
@@ -444,7 +446,7 @@ def roofline(w, leg, profile_key):
     if traffic:
         out["traffic_note"] = ("HBM bytes per working k_cg_iter launch from the committed rocprofv3 "
                                "FETCH_SIZE / WRITE_SIZE summary (profiles/), scaled by a copy of known "
-                               "size; not measured in this run")
+                               f"size; not measured in this run; profiled build: {traffic_build}")
         if "avg_launch_us" in out:
             out["counted_frac"] = traffic / (out["avg_launch_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS
         elif profile_avg_us:
@@ -454,7 +456,8 @@ def roofline(w, leg, profile_key):
                                    "trace's mean duration, both from the committed profiles")
     if profile_avg_us:
         out["profile_avg_launch_us"] = profile_avg_us
-        out["profile_note"] = "committed rocprofv3 kernel trace (profiles/), not measured in this run"
+        out["profile_note"] = ("committed rocprofv3 kernel trace (profiles/), not measured in this run; "
+                               f"profiled build: {profile_build}")
     return out
 
 
