@@ -60,10 +60,32 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
   return xcd * q + (xcd < r ? xcd : r) + k;
 }
 
+// One DPP move of a double (two 32-bit halves); lanes the row mask leaves
+// out receive 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 64 lanes (called with every lane active), in a fixed order,
+// returned to every lane.  Cross-lane steps are DPP moves (no LDS): pairs
+// and quads (quad_perm), half rows and rows (mirrors), then row 1 += row 0
+// and row 3 += row 2 (row_bcast:15), row 3 += rows 0 + 1 (row_bcast:31);
+// lane 63 ends with the total.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += dpp_f64<0xb1, 0xf>(v);   // quad_perm [1, 0, 3, 2]
+  v += dpp_f64<0x4e, 0xf>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_f64<0x141, 0xf>(v);  // row_half_mirror
+  v += dpp_f64<0x140, 0xf>(v);  // row_mirror
+  v += dpp_f64<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_f64<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 // Deterministic block sum (fixed tree), result broadcast to every thread.
