@@ -88,29 +88,26 @@ __device__ __forceinline__ double wave_sum(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Deterministic block sum (fixed tree), result broadcast to every thread.
-// `red` holds BLOCK/64 + 1 doubles of LDS.
-template <int BLOCK>
+// Deterministic block sum (fixed tree), result broadcast to every thread:
+// after one barrier every thread adds the waves' sums itself, in wave order.
+// `red` holds BLOCK/64 + 1 doubles of LDS.  kReuse adds a trailing barrier,
+// so that `red` may be written again right after the call.
+template <int BLOCK, bool kReuse = true>
 __device__ __forceinline__ double block_sum(double a, double *red) {
   constexpr int W = BLOCK / 64;
   a = wave_sum(a);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   if (l == 0) red[w] = a;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int i = 0; i < W; ++i) s += red[i];
-    red[W] = s;
-  }
-  __syncthreads();
-  double r = red[W];
-  __syncthreads();
+  double r = 0.0;
+  for (int i = 0; i < W; ++i) r += red[i];
+  if (kReuse) __syncthreads();
   return r;
 }
 
-// Four sums at once (one tree, one set of barriers); result broadcast.
-// `red4` holds 4 * (BLOCK/64 + 1) doubles of LDS.
-template <int BLOCK>
+// Four sums at once (one tree, one barrier); result broadcast.  `red4`
+// holds 4 * (BLOCK/64 + 1) doubles of LDS; kReuse as for block_sum.
+template <int BLOCK, bool kReuse = true>
 __device__ __forceinline__ double4 block_sum4(double4 a, double *red4) {
   constexpr int W = BLOCK / 64;
   a.x = wave_sum(a.x);
@@ -125,14 +122,14 @@ __device__ __forceinline__ double4 block_sum4(double4 a, double *red4) {
     red4[4 * w + 3] = a.w;
   }
   __syncthreads();
-  if (threadIdx.x < 4) {
-    double s = 0.0;
-    for (int i = 0; i < W; ++i) s += red4[4 * i + threadIdx.x];
-    red4[4 * W + threadIdx.x] = s;
+  double4 r = make_double4(0.0, 0.0, 0.0, 0.0);
+  for (int i = 0; i < W; ++i) {
+    r.x += red4[4 * i];
+    r.y += red4[4 * i + 1];
+    r.z += red4[4 * i + 2];
+    r.w += red4[4 * i + 3];
   }
-  __syncthreads();
-  double4 r = make_double4(red4[4 * W], red4[4 * W + 1], red4[4 * W + 2], red4[4 * W + 3]);
-  __syncthreads();
+  if (kReuse) __syncthreads();
   return r;
 }
 
@@ -1033,7 +1030,9 @@ void k_cg_iter(StepArgs A, int it) {
     part.w += q.w;
   }
   if (kFixed && threadIdx.x == 0) s_pmax = 0ull;  // ordered by block_sum4's barriers
-  const double4 s4 = block_sum4<BLOCK>(part, red);
+  // no trailing barrier: `red` is next written after the barriers around
+  // the slot loop
+  const double4 s4 = block_sum4<BLOCK, false>(part, red);
   double alpha = 0.0, beta = 0.0, rr, bb;
   if (it == 0) {
     rr = s4.x;
@@ -1165,7 +1164,7 @@ void k_cg_iter(StepArgs A, int it) {
     }
     if (m >= 0.0) node_partials<Acc, PT>(A, ti.node_begin + l, l, m, sp, sr, sy, Yn, acc, inv);
   }
-  const double4 tot = block_sum4<BLOCK>(acc, red);
+  const double4 tot = block_sum4<BLOCK, false>(acc, red);
   if (threadIdx.x == 0) reinterpret_cast<double4 *>(A.part4 + cur * 4 * (int64_t)nt)[tile] = tot;
 }
 
