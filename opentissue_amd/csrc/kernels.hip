@@ -1434,6 +1434,14 @@ __global__ __launch_bounds__(kSubGramBlock) void k_sub_gram(StepArgs A) {
 // scaled to unit diagonal; Cholesky by columns with directions dropped whose
 // pivot falls below 1e-12 (numerically dependent on the earlier ones, or
 // zero); dropped directions get c = 0.  out[kSubMax] receives c.
+// Lane j's value of x to every lane (j wave-uniform: a scalar read, no LDS).
+__device__ __forceinline__ double lane_f64(double x, int j) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, j);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 __device__ void sub_solve_wave(const double *gram, int k, double *out) {
   const int lane = threadIdx.x & 63;
   const bool in = lane < k;
@@ -1443,7 +1451,7 @@ __device__ void sub_solve_wave(const double *gram, int k, double *out) {
   double row[kSubMax];
 #pragma unroll
   for (int j = 0; j < kSubMax; ++j) {
-    const double sj = __shfl(si, j, 64);
+    const double sj = lane_f64(si, j);
     row[j] = (in && j < k) ? gram[sub_pair(min(lane, j), max(lane, j))] * si * sj : 0.0;
   }
   const double bi = in ? gram[kSubPairs + lane] * si : 0.0;
@@ -1457,7 +1465,7 @@ __device__ void sub_solve_wave(const double *gram, int k, double *out) {
 #pragma unroll
       for (int m = 0; m < j; ++m)
         if ((keep >> m) & 1u) d -= row[m] * row[m];
-      d = __shfl(d, j, 64);
+      d = lane_f64(d, j);
       if (!(d > 1e-12)) {
         keep &= ~(1u << j);
       } else {
@@ -1466,7 +1474,7 @@ __device__ void sub_solve_wave(const double *gram, int k, double *out) {
         double v = row[j];
 #pragma unroll
         for (int m = 0; m < j; ++m)
-          if ((keep >> m) & 1u) v -= row[m] * __shfl(row[m], j, 64);
+          if ((keep >> m) & 1u) v -= row[m] * lane_f64(row[m], j);
         if (lane > j)
           row[j] = v / ljj;
         else if (lane == j)
@@ -1483,7 +1491,7 @@ __device__ void sub_solve_wave(const double *gram, int k, double *out) {
 #pragma unroll
       for (int m = 0; m < j; ++m)
         if ((keep >> m) & 1u) v -= row[m] * y[m];
-      y[j] = __shfl(v, j, 64) / ldiag[j];
+      y[j] = lane_f64(v, j) / ldiag[j];
     }
   }
   const bool mine = lane < kSubMax && ((keep >> (lane < kSubMax ? lane : 0)) & 1u);
