@@ -507,3 +507,22 @@ def test_stop_measures_the_true_residual(mode):
         print(mode, step, st.iterations, res, np.sqrt(st.rr / st.bb), err)
         assert st.iterations > 100 and st.bodies_converged == 1, (step, st.iterations)
         assert res <= bound and err <= TOL, (step, res, err)
+
+
+def test_modes_alternate_on_one_solver():
+    """The CG direction's storage follows the step's settings (FP32 with a
+    relative-residual stop, FP64 in the reference mode's fixed 20
+    iterations), in one buffer of the solver: steps that alternate the two
+    modes on one solver each match the oracle from a shared state (per-step
+    displacement error <= 1e-6; the relative-residual steps solved to 1e-8)."""
+    from opentissue_amd import fem
+    import oracle
+    c, t, fx = _lattice(4, 10, 3)
+    s, o = _pair(c, t, fx)
+    modes = [(fem.reference_params(), oracle.reference_params()),
+             (fem.config_params(tol=1e-8, cap=3000), oracle.config_params(tol=1e-8, cap=3000))]
+    for step in range(6):
+        gp, op = modes[step % 2]
+        errs, iters = _per_step_errors(s, o, c, 1, 0.01, gp, op)
+        print(step, errs, iters)
+        assert errs.max() <= TOL, (step, errs, iters)
