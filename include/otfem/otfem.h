@@ -206,7 +206,7 @@ otfem_status otfem_set_kernel_config(otfem_handle h, int cg_block, int spmv_mode
  * the same kernel configuration, gives the same bits on every run (the dot
  * products are fixed-order trees in both modes).  CG arithmetic is FP64 in
  * this mode.  It costs a buffer of 288 B per slot and about 1.1x the step
- * time (64^3 cube: 8.92 ms against 8.12 ms).  No reference counterpart (the
+ * time (64^3 cube: 8.29 ms against 7.42 ms).  No reference counterpart (the
  * reference is single-threaded, hence deterministic). */
 otfem_status otfem_set_deterministic(otfem_handle h, int on);
 /* Autotuner: times `steps` steps with each CG block size (256/512/1024
