@@ -19,6 +19,7 @@ struct BodyState {
   int32_t stopped;  // 0 while running; else 1 + index of the CG kernel that recorded the stop
   int32_t converged;
   int32_t flags;  // bit 0: non-finite scalar
+  double alpha_hist[2];  // alpha of iteration j-1, written by (even) CG kernel j at [j & 1]
 };
 
 // SpMV arithmetic inside CG (element data is FP32 in memory in every mode).

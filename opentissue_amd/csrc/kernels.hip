@@ -806,6 +806,12 @@ struct CgShape {
   // loaded with the other early loads (with 128 registers per thread: stop
   // test first, partials after the node inputs)
   static constexpr bool kEarlyPartials = BLOCK <= 256;
+  // even kernels leave v alone and the next (odd) kernel adds both
+  // iterations' updates: 48 B less per owned node in every other kernel for
+  // 12 B more in the odd ones (-4.2% CG time on the 2,048-body set).  Not at
+  // 512 and 1024 threads, whose 128 registers would spill the extra loads
+  // (the 64^3 cube, latency-bound at 512 threads, measured 7.6% slower).
+  static constexpr bool kDeferV = BLOCK <= 256;
 };
 
 // One tile node's inputs to the CG prologue, as loaded (PT: the storage
@@ -817,16 +823,20 @@ struct NodeIn {
   double m;
   double r[3], y[3], v[3];
   PT p[3];
+  PT pp[3];  // p of two iterations back (odd kernels from 3 on, owned nodes)
 };
 
 // Loads the inputs of tile nodes l = base + threadIdx.x + k*BLOCK (k < NB)
 // without using any of them, so every load of the batch is in flight at
 // once.  Lanes past the node table read node_begin (in bounds) and are never
-// used.
-template <int BLOCK, int NB, typename PT>
+// used.  With kDefer, v (and, from kernel 3 on, the p of two iterations
+// back, still in the buffer this kernel writes p to) is loaded by odd
+// kernels only: even kernels defer their update of v to the next kernel.
+template <int BLOCK, int NB, typename PT, bool kDefer>
 __device__ __forceinline__ void load_nodes(const StepArgs &A, const TileInfo &ti, int it,
                                            int base, const double *Rp, const PT *Pp,
-                                           const double *Yp, NodeIn<PT> (&nd)[NB]) {
+                                           const double *Yp, const PT *Pq,
+                                           NodeIn<PT> (&nd)[NB]) {
   const int nn = ti.n_owned + ti.n_halo, no = ti.n_owned;
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
@@ -852,7 +862,14 @@ __device__ __forceinline__ void load_nodes(const StepArgs &A, const TileInfo &ti
       for (int d = 0; d < 3; ++d) {
         nd[k].p[d] = Pp[3 * g + d];
         nd[k].y[d] = Yp[3 * g + d];
-        nd[k].v[d] = l < no ? A.v[3 * g + d] : 0.0;
+      }
+      if (!kDefer || (it & 1)) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) nd[k].v[d] = l < no ? A.v[3 * g + d] : 0.0;
+      }
+      if (kDefer && (it & 1) && it >= 3) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) nd[k].pp[d] = l < no ? Pq[3 * g + d] : (PT)0;
       }
     }
   }
@@ -865,8 +882,13 @@ __device__ __forceinline__ void load_nodes(const StepArgs &A, const TileInfo &ti
 // nodes carry zero r and p.  p is rounded to its storage type PT before any
 // use: the SpMV, the dot products and (next kernel) v += alpha p and
 // r -= alpha A p all see the same vector, so r stays the residual of v.
-template <typename AT, typename PT>
+// With kDefer, v is read and written by odd kernels only: kernel it (odd,
+// from 3 on) adds alpha_prev p(it-2) and then alpha p(it-1), in the order
+// single updates would; an even kernel updates v only when it records the
+// body's stop.
+template <typename AT, typename PT, bool kDefer>
 __device__ __forceinline__ void update_node(const StepArgs &A, int it, bool stop, double alpha,
+                                            double alpha_prev,
                                             double beta, int l, int no, const NodeIn<PT> &n,
                                             double *Rn, PT *Pn, PT *sp, double *sr,
                                             AT *sy, double *sm, double &pmax) {
@@ -886,8 +908,17 @@ __device__ __forceinline__ void update_node(const StepArgs &A, int it, bool stop
       pn[d] = rn[d] + beta * n.p[d];
     }
     if (l < no) {
+      if (!kDefer || (it & 1)) {
 #pragma unroll
-      for (int d = 0; d < 3; ++d) A.v[3 * g + d] = n.v[d] + alpha * n.p[d];
+        for (int d = 0; d < 3; ++d) {
+          double v = n.v[d];
+          if (kDefer && it >= 3) v = v + alpha_prev * n.pp[d];
+          A.v[3 * g + d] = v + alpha * n.p[d];
+        }
+      } else if (stop) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) A.v[3 * g + d] = A.v[3 * g + d] + alpha * n.p[d];
+      }
     }
   }
   PT pf[3];
@@ -991,6 +1022,10 @@ void k_cg_iter(StepArgs A, int it) {
   PT *Pn = reinterpret_cast<PT *>(A.p) + cur * 3 * A.N;
   const double *Rp = A.r + prv * 3 * A.N, *Yp = A.y + prv * 3 * A.N;
   const PT *Pp = reinterpret_cast<const PT *>(A.p) + prv * 3 * A.N;
+  // alpha of iteration it-2, whose update of v the previous (even) kernel
+  // deferred to this one
+  constexpr bool kDefer = CgShape<BLOCK>::kDeferV;
+  const double alpha_prev = (kDefer && (it & 1) && it >= 3) ? bs->alpha_hist[prv] : 0.0;
   // tile partials: r.r and b.b of the element kernel at it == 0, else
   // p.q, r.q, q.q, r.r of the previous kernel
   const double4 *prevp = reinterpret_cast<const double4 *>(A.part4 + prv * 4 * (int64_t)nt);
@@ -1012,7 +1047,7 @@ void k_cg_iter(StepArgs A, int it) {
       load_raw<ST>(A, S, min(sb + (int)threadIdx.x + u * BLOCK, send - 1), ra[u]);
   }
   NodeIn<PT> nd[NB];
-  load_nodes<BLOCK, NB, PT>(A, ti, it, 0, Rp, Pp, Yp, nd);
+  load_nodes<BLOCK, NB, PT, kDefer>(A, ti, it, 0, Rp, Pp, Yp, Pn, nd);
 
   // exit if the body stopped in an earlier kernel (a stop recorded by this
   // kernel still needs this workgroup's prologue: v += alpha p)
@@ -1045,8 +1080,20 @@ void k_cg_iter(StepArgs A, int it) {
     alpha = rrd / pq;
     if (!finite(alpha)) {
       if (ti.body_rank == 0 && threadIdx.x == 0) record_stop(A, bs, it - 1, rrd, false, true, it);
+      // v keeps the iterate of iteration it-2: add the update deferred to
+      // this kernel
+      if (kDefer && (it & 1) && it >= 3) {
+        for (int l = threadIdx.x; l < no; l += BLOCK) {
+          const int64_t g = ti.node_begin + l;
+          if (A.fixed[g]) continue;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) A.v[3 * g + d] = A.v[3 * g + d] + alpha_prev * Pn[3 * g + d];
+        }
+      }
       return;
     }
+    // the next (odd) kernel completes this kernel's update of v
+    if (kDefer && !(it & 1) && ti.body_rank == 0 && threadIdx.x == 0) bs->alpha_hist[cur] = alpha;
     rr = fmax(rrd - 2.0 * alpha * rq + alpha * alpha * qq, 0.0);
     double d = rrd;
     if (fabs(d) < A.tiny) d = A.tiny;  // fem_conjugate_gradients.h:121-122
@@ -1079,11 +1126,12 @@ void k_cg_iter(StepArgs A, int it) {
     for (int k = 0; k < NB; ++k) {
       const int l = base + (int)threadIdx.x + k * BLOCK;
       if (l < nn)
-        update_node<Acc, PT>(A, it, stop, alpha, beta, l, no, nd[k], Rn, Pn, sp, sr, sy, sm, pmax);
+        update_node<Acc, PT, kDefer>(A, it, stop, alpha, alpha_prev, beta, l, no, nd[k], Rn, Pn, sp, sr,
+                             sy, sm, pmax);
     }
     base += NB * BLOCK;
     if (base >= nn) break;
-    load_nodes<BLOCK, NB, PT>(A, ti, it, base, Rp, Pp, Yp, nd);
+    load_nodes<BLOCK, NB, PT, kDefer>(A, ti, it, base, Rp, Pp, Yp, Pn, nd);
   }
   if (stop) return;
   if (kFixed && pmax > 0.0) atomicMax(&s_pmax, (unsigned long long)__double_as_longlong(pmax));
@@ -1683,8 +1731,11 @@ hipError_t enqueue_subspace(const StepArgs &a, size_t lds_gram, size_t lds_start
 }
 
 // Bodies still running after cg_max iterations complete their last update
-// here (v += alpha p) and record their stop; then x += dt v
-// (fem_position_update.h:33-38).
+// here (v += alpha p, and the update kernel cg_max-1 deferred when it was an
+// even kernel of a deferring shape, CgShape::kDeferV) and record their stop; then x += dt v
+// (fem_position_update.h:33-38).  This kernel counts as CG kernel cg_max: it
+// records the stop as 1 + cg_max, so workgroups of the body that start after
+// the body's first tile has recorded it still complete their nodes.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_finish(StepArgs A) {
   __shared__ double red[4 * (BLOCK / 64 + 1)];
@@ -1693,7 +1744,8 @@ __global__ __launch_bounds__(BLOCK) void k_finish(StepArgs A) {
   BodyState *bs = &A.body[ti.body];
   const int stopped = *(volatile int32_t *)&bs->stopped;
   const int it = A.cg_max;
-  if (!stopped && it > 0) {
+  const bool running = stopped == 0 || stopped - 1 >= it;
+  if (running && it > 0) {
     const int t0 = ti.body_tile_begin, t1 = ti.body_tile_end;
     const double *prevp = A.part4 + ((it - 1) & 1) * 4 * (int64_t)A.ntiles;
     const double4 s4 = body_sum4<BLOCK>(reinterpret_cast<const double4 *>(prevp), t0, t1, red);
@@ -1702,16 +1754,24 @@ __global__ __launch_bounds__(BLOCK) void k_finish(StepArgs A) {
     if (fabs(pq) < A.tiny) pq = A.tiny;
     const double alpha = rrd / pq;
     const bool ok = finite(alpha);
-    if (ok) {
-      const int64_t off = ((it - 1) & 1) * 3 * A.N;
-      const double *Pp = A.p + off;
-      const float *Pf = reinterpret_cast<const float *>(A.p) + off;
+    // kernel it-1 was even (it odd, from 3 on): it deferred alpha(it-2) p(it-2)
+    const bool two = CgShape<BLOCK>::kDeferV && (it & 1) && it >= 3;
+    const double alpha_prev = two ? bs->alpha_hist[(it - 1) & 1] : 0.0;
+    if (ok || two) {
+      const int64_t off1 = ((it - 1) & 1) * 3 * A.N, off2 = (it & 1) * 3 * A.N;
+      const double *P1 = A.p + off1, *P2 = A.p + off2;
+      const float *F1 = reinterpret_cast<const float *>(A.p) + off1;
+      const float *F2 = reinterpret_cast<const float *>(A.p) + off2;
       for (int l = threadIdx.x; l < ti.n_owned; l += BLOCK) {
         const int64_t g = ti.node_begin + l;
         if (A.fixed[g]) continue;
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
-          A.v[3 * g + d] += alpha * (A.p32 ? (double)Pf[3 * g + d] : Pp[3 * g + d]);
+        for (int d = 0; d < 3; ++d) {
+          double v = A.v[3 * g + d];
+          if (two) v = v + alpha_prev * (A.p32 ? (double)F2[3 * g + d] : P2[3 * g + d]);
+          if (ok) v = v + alpha * (A.p32 ? (double)F1[3 * g + d] : P1[3 * g + d]);
+          A.v[3 * g + d] = v;
+        }
       }
     }
     if (ti.body_rank == 0 && threadIdx.x == 0) {
@@ -1722,10 +1782,10 @@ __global__ __launch_bounds__(BLOCK) void k_finish(StepArgs A) {
       bs->rr = rr;
       bs->converged = conv ? 1 : 0;
       bs->flags |= (nonfin || !ok) ? 1 : 0;
-      bs->stopped = 1;
+      bs->stopped = it + 1;
     }
     __syncthreads();
-  } else if (!stopped && ti.body_rank == 0 && threadIdx.x == 0) {
+  } else if (running && ti.body_rank == 0 && threadIdx.x == 0) {
     bs->iterations = 0;
     bs->rr = bs->rr0;
     bs->stopped = 1;
