@@ -526,3 +526,21 @@ def test_modes_alternate_on_one_solver():
         errs, iters = _per_step_errors(s, o, c, 1, 0.01, gp, op)
         print(step, errs, iters)
         assert errs.max() <= TOL, (step, errs, iters)
+
+
+def test_capped_steps_with_more_tiles_than_resident_workgroups():
+    """Reference-mode steps end at the iteration cap, so every body
+    completes its last update in k_finish.  With 1024-thread tiles (one
+    workgroup per CU) and 621 tiles, workgroups of the body start after its
+    first tile has recorded the stop; they still complete their nodes (the
+    stop is recorded as that of kernel cg_max), and each step matches the
+    oracle from a shared state."""
+    from opentissue_amd import fem
+    import oracle
+    c, t, fx = _lattice(10, 40, 10)
+    s, o = _pair(c, t, fx, target_tile_nodes=8)
+    s.set_kernel_config(cg_block=1024)
+    assert s.layout_info()["tiles"] > 256
+    errs, iters = _per_step_errors(s, o, c, 3, 0.01, fem.reference_params(), oracle.reference_params())
+    print(s.layout_info()["tiles"], errs, iters)
+    assert errs.max() <= TOL, errs
