@@ -38,7 +38,7 @@ struct StepArgs {
   double *r;             // [2][3N] residual, alternating by CG iteration
   double *p;             // [2][3N] search direction (FP64), or [2][3N] FP32 in its first half
                          //         with `p32`
-  double *y;             // [2][3N] K' p
+  double *y;             // [2][3N] q = A p = cM m p + cK K' p (0 at fixed nodes)
   // tiles / slots
   const TileInfo *tiles;
   const int32_t *halo;

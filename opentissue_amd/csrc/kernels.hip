@@ -854,7 +854,9 @@ __device__ __forceinline__ void load_nodes(const StepArgs &A, const TileInfo &ti
     const int64_t g = nd[k].g;
     const int l = base + (int)threadIdx.x + k * BLOCK;
     nd[k].fx = A.fixed[g];
-    nd[k].m = A.mass[g];
+    // the mass enters q only, which the previous kernel stored: owned nodes
+    // need it for this kernel's q, halo nodes not at all
+    nd[k].m = l < no ? A.mass[g] : 0.0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) nd[k].r[d] = R[3 * g + d];
     if (it > 0) {
@@ -876,10 +878,10 @@ __device__ __forceinline__ void load_nodes(const StepArgs &A, const TileInfo &ti
 }
 
 // Completes iteration it-1 for one tile node and forms the new direction:
-// r = r - alpha q, p = r + beta p (q = cM m p + cK y), v += alpha p for owned
-// nodes; p goes to LDS for the slot loop, r and p of owned nodes to LDS and
-// memory, and the mass of owned nodes to LDS (-1 marks a fixed node).  Fixed
-// nodes carry zero r and p.  p is rounded to its storage type PT before any
+// r = r - alpha q, p = r + beta p (q = A p as the previous kernel stored
+// it), v += alpha p for owned nodes; p goes to LDS for the slot loop, r and
+// p of owned nodes to LDS and memory, and the mass of owned nodes to LDS (-1
+// marks a fixed node).  Fixed nodes carry zero r and p.  p is rounded to its storage type PT before any
 // use: the SpMV, the dot products and (next kernel) v += alpha p and
 // r -= alpha A p all see the same vector, so r stays the residual of v.
 // With kDefer, v is read and written by odd kernels only: kernel it (odd,
@@ -903,8 +905,7 @@ __device__ __forceinline__ void update_node(const StepArgs &A, int it, bool stop
   } else {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const double q = A.cM * n.m * n.p[d] + A.cK * n.y[d];
-      rn[d] = n.r[d] - alpha * q;
+      rn[d] = n.r[d] - alpha * n.y[d];  // y holds q = A p of the previous kernel
       pn[d] = rn[d] + beta * n.p[d];
     }
     if (l < no) {
@@ -946,7 +947,8 @@ __device__ __forceinline__ void update_node(const StepArgs &A, int it, bool stop
 }
 
 // Owned node l's contribution to the tile partials p.q, r.q, q.q, r.r, and
-// its K'p to memory.
+// its q = A p = cM m p + cK K'p to memory (the next kernel's r -= alpha q
+// uses the same value).
 template <typename AT, typename PS>
 __device__ __forceinline__ void node_partials(const StepArgs &A, int64_t g, int l, double m,
                                               const PS *sp, const double *sr, const AT *sy,
@@ -954,9 +956,9 @@ __device__ __forceinline__ void node_partials(const StepArgs &A, int64_t g, int 
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const double yd = acc_value<AT>(sy[3 * l + d], inv);
-    Yn[3 * g + d] = yd;
     const double p = sp[3 * l + d], r = sr[3 * l + d];
     const double q = A.cM * m * p + A.cK * yd;
+    Yn[3 * g + d] = q;
     acc.x += p * q;
     acc.y += r * q;
     acc.z += q * q;
