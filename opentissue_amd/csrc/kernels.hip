@@ -820,8 +820,8 @@ template <typename PT>
 struct NodeIn {
   int32_t g;  // internal node id
   int32_t fx;
-  double m;
-  double r[3], y[3], v[3];
+  double m;  // owned nodes only
+  double r[3], y[3], v[3];  // y: q = A p as the previous kernel stored it
   PT p[3];
   PT pp[3];  // p of two iterations back (odd kernels from 3 on, owned nodes)
 };
