@@ -1268,25 +1268,39 @@ __device__ __forceinline__ SubState sub_state(const StepArgs &A) {
   return s;
 }
 
-// Direction j >= 1 at internal node g as FP32: d_1 = v^n - v^(n-1) (given),
-// the older differences from the ring, the network's start - v^n.
-__device__ __forceinline__ void sub_dir(const StepArgs &A, int64_t g, const SubState &q, int j,
-                                        const float d1[3], const double vn[3], float d[3]) {
-  if (j == 1 && q.nh >= 1) {
-    d[0] = d1[0];
-    d[1] = d1[1];
-    d[2] = d1[2];
-  } else if (j <= q.nh) {
-    int slot = q.head - j + 1;
-    if (slot < 0) slot += A.sub_ring;
-    const float4 h = A.vring[(int64_t)slot * A.N + g];
-    d[0] = h.x;
-    d[1] = h.y;
-    d[2] = h.z;
-  } else {
+// The loaded directions of one node: ring entries h[j] for j = 2 .. kSubMax - 1
+// (entries 0 and 1 unused) and the network's start vt.
+struct SubDirs {
+  float4 h[kSubMax];
+  double vt[3];
+};
+
+// Issues every load of node g's directions at once, with no branch, so a
+// node waits for memory once: ring slots past those the step uses repeat the
+// last one used, and without the network vt reads v^n.
+__device__ __forceinline__ void sub_load(const StepArgs &A, int64_t g, const SubState &q,
+                                         SubDirs &s) {
+  const int jmax = max(min(q.nh, q.k - 1), 2);
 #pragma unroll
-    for (int x = 0; x < 3; ++x) d[x] = (float)(A.vnet[3 * g + x] - vn[x]);
+  for (int j = 2; j < kSubMax; ++j) {
+    int slot = q.head - min(j, jmax) + 1;
+    if (slot < 0) slot += A.sub_ring;
+    s.h[j] = A.vring[(int64_t)slot * A.N + g];
   }
+  const double *vp = A.sub_net ? A.vnet : A.v;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) s.vt[x] = vp[3 * g + x];
+}
+
+// Component x of direction j >= 1 (j < k) at node g as FP32: d_1 = v^n -
+// v^(n-1) (given), the older differences from the ring, the network's start
+// - v^n.
+__device__ __forceinline__ float sub_get(const SubDirs &s, const SubState &q, int j, int x,
+                                         const float d1[3], const double vn[3]) {
+  const float net = (float)(s.vt[x] - vn[x]);
+  if (j == 1) return q.nh >= 1 ? d1[x] : net;
+  const float hv = x == 0 ? s.h[j].x : (x == 1 ? s.h[j].y : s.h[j].z);
+  return j <= q.nh ? hv : net;
 }
 
 // Scale factors of E~ = L eps with L^T L = cK D: eps^T D eps =
@@ -1317,32 +1331,51 @@ __global__ __launch_bounds__(kSubGramBlock) void k_sub_gram(StepArgs A) {
   const double *vprev = A.vbuf + (int64_t)q.par * 3 * A.N;
   double *vcur = A.vbuf + (int64_t)(1 - q.par) * 3 * A.N;
   const double cK = A.cK, cM = A.cM;
-  for (int l = threadIdx.x; l < nn; l += kSubGramBlock) {
-    const int64_t g = l < no ? ti.node_begin + l : A.halo[ti.halo_begin + l - no];
-    const double vn[3] = {A.v[3 * g], A.v[3 * g + 1], A.v[3 * g + 2]};
-    const bool fx = A.fixed[g] != 0;
-    float d1[3] = {0.f, 0.f, 0.f};
-    if (q.prev >= 1)
+  // two nodes per thread and trip (a tile has fewer than two nodes per
+  // thread): both nodes' loads are issued before either is used; the second
+  // node of a trip past the table repeats the first and stores nothing
+  for (int l0 = threadIdx.x; l0 < nn; l0 += 2 * kSubGramBlock) {
+    const int ln[2] = {l0, l0 + kSubGramBlock < nn ? l0 + kSubGramBlock : l0};
+    int64_t g[2];
 #pragma unroll
-      for (int x = 0; x < 3; ++x) d1[x] = (float)(vn[x] - vprev[3 * g + x]);
-    // every direction's loads are issued before any is stored
-    float d[kSubMax][3];
+    for (int h = 0; h < 2; ++h)
+      g[h] = ln[h] < no ? ti.node_begin + ln[h] : A.halo[ti.halo_begin + ln[h] - no];
+    double vn[2][3], vp[2][3];
+    bool fx[2];
+    SubDirs sd[2];
 #pragma unroll
-    for (int x = 0; x < 3; ++x) d[0][x] = (float)vn[x];
+    for (int h = 0; h < 2; ++h) {
 #pragma unroll
-    for (int j = 1; j < kSubMax; ++j)
-      if (j < k) sub_dir(A, g, q, j, d1, vn, d[j]);
-    if (l < no) {
-#pragma unroll
-      for (int x = 0; x < 3; ++x) vcur[3 * g + x] = vn[x];
-      if (q.prev >= 1) A.vring[(int64_t)q.head * A.N + g] = make_float4(d1[0], d1[1], d1[2], 0.f);
+      for (int x = 0; x < 3; ++x) {
+        vn[h][x] = A.v[3 * g[h] + x];
+        vp[h][x] = vprev[3 * g[h] + x];
+      }
+      fx[h] = A.fixed[g[h]] != 0;
+      sub_load(A, g[h], q, sd[h]);
     }
-    float *o = sdir + (int64_t)l * k * 3;
 #pragma unroll
-    for (int j = 0; j < kSubMax; ++j)
-      if (j < k)
+    for (int h = 0; h < 2; ++h) {
+      if (h == 1 && ln[1] == ln[0]) break;
+      const int l = ln[h];
+      float d1[3] = {0.f, 0.f, 0.f};
+      if (q.prev >= 1)
 #pragma unroll
-        for (int x = 0; x < 3; ++x) o[3 * j + x] = fx ? 0.f : d[j][x];
+        for (int x = 0; x < 3; ++x) d1[x] = (float)(vn[h][x] - vp[h][x]);
+      if (l < no) {
+#pragma unroll
+        for (int x = 0; x < 3; ++x) vcur[3 * g[h] + x] = vn[h][x];
+        if (q.prev >= 1)
+          A.vring[(int64_t)q.head * A.N + g[h]] = make_float4(d1[0], d1[1], d1[2], 0.f);
+      }
+      float *o = sdir + l * k * 3;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) o[x] = fx[h] ? 0.f : (float)vn[h][x];
+#pragma unroll
+      for (int j = 1; j < kSubMax; ++j)
+        if (j < k)
+#pragma unroll
+          for (int x = 0; x < 3; ++x) o[3 * j + x] = fx[h] ? 0.f : sub_get(sd[h], q, j, x, d1, vn[h]);
+    }
   }
   __syncthreads();
 
@@ -1434,26 +1467,44 @@ __global__ __launch_bounds__(kSubGramBlock) void k_sub_gram(StepArgs A) {
       if (base - qs + 2 * U * R >= pe) break;
     }
   }
-  // mass part (rows sqrt(cM m) d_x) and D^T r_plain over owned free nodes
+  // mass part (rows sqrt(cM m) d_x) and D^T r_plain over owned free nodes:
+  // four rounds of 4 nodes per wave at a time, their loads issued together
+  // (rows of lanes past the k directions or the owned nodes, and of fixed
+  // nodes, are zero)
   double gacc = 0.0;
-  for (int nb = 4 * wave; nb < no; nb += 4 * W) {
-    const int l = nb + qs;
-    double e[3] = {0.0, 0.0, 0.0};
-    if (l < no && live) {
-      const int64_t g = ti.node_begin + l;
-      if (!A.fixed[g]) {
-        const double sq = sqrt(cM * A.mass[g]);
-        const float *d = sdir + ((int64_t)l * k + dir) * 3;
+  const int dc = min(dir, k - 1);
+  for (int nb = 4 * wave; nb < no; nb += 16 * W) {
+    bool ok[4];
+    double m[4], rv[4][3];
+    int lc[4];
 #pragma unroll
-        for (int x = 0; x < 3; ++x) {
-          e[x] = sq * (double)d[x];
-          gacc += (double)d[x] * A.r[3 * g + x];
-        }
-      }
+    for (int t = 0; t < 4; ++t) {
+      const int l = nb + 4 * W * t + qs;
+      lc[t] = min(l, no - 1);
+      const int64_t g = ti.node_begin + lc[t];
+      const bool free_node = A.fixed[g] == 0;
+      ok[t] = (l < no) & live & free_node;  // no short circuit: every load is issued
+      m[t] = A.mass[g];
+#pragma unroll
+      for (int x = 0; x < 3; ++x) rv[t][x] = A.r[3 * g + x];
     }
-    C0 = __builtin_amdgcn_mfma_f64_16x16x4f64(e[0], e[0], C0, 0, 0, 0);
-    C1 = __builtin_amdgcn_mfma_f64_16x16x4f64(e[1], e[1], C1, 0, 0, 0);
-    C0 = __builtin_amdgcn_mfma_f64_16x16x4f64(e[2], e[2], C0, 0, 0, 0);
+    // the masses are used below only where a node is free; this keeps their
+    // loads unconditional and issued with the others
+    asm volatile("" : "+v"(m[0]), "+v"(m[1]), "+v"(m[2]), "+v"(m[3]));
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const double sq = sqrt(cM * (ok[t] ? m[t] : 0.0));  // fixed nodes hold DBL_MAX
+      const float *d = sdir + (lc[t] * k + dc) * 3;
+      double e[3];
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        e[x] = sq * (double)d[x];
+        gacc += ok[t] ? (double)d[x] * rv[t][x] : 0.0;
+      }
+      C0 = __builtin_amdgcn_mfma_f64_16x16x4f64(e[0], e[0], C0, 0, 0, 0);
+      C1 = __builtin_amdgcn_mfma_f64_16x16x4f64(e[1], e[1], C1, 0, 0, 0);
+      C0 = __builtin_amdgcn_mfma_f64_16x16x4f64(e[2], e[2], C0, 0, 0, 0);
+    }
   }
   gacc += __shfl_xor(gacc, 16, 64);
   gacc += __shfl_xor(gacc, 32, 64);
@@ -1600,38 +1651,39 @@ __global__ __launch_bounds__(BLOCK) void k_sub_start(StepArgs A) {
   const int k = q.k;
   const int nn = ti.n_owned + ti.n_halo, no = ti.n_owned;
   const double *vcur = A.vbuf + (int64_t)(1 - q.par) * 3 * A.N;  // v^n
-  const double *cf = A.sub_coef + (int64_t)ti.body * kSubMax;
+  // the body's coefficients, loaded once (every entry is written, 0 past k)
+  double cf[kSubMax];
+#pragma unroll
+  for (int j = 0; j < kSubMax; ++j) cf[j] = A.sub_coef[(int64_t)ti.body * kSubMax + j];
   double *sp = lds;         // w = D c at every tile node
   double *sy = sp + 3 * nn; // K' w at owned nodes
   for (int l = threadIdx.x; l < nn; l += BLOCK) {
     const int64_t g = l < no ? ti.node_begin + l : A.halo[ti.halo_begin + l - no];
-    double wv[3] = {0.0, 0.0, 0.0};
-    if (!A.fixed[g]) {
-      const double vn[3] = {vcur[3 * g], vcur[3 * g + 1], vcur[3 * g + 2]};
-      float d1[3] = {0.f, 0.f, 0.f};
-      if (q.nh >= 1) {
-        const float4 h = A.vring[(int64_t)q.head * A.N + g];
-        d1[0] = h.x;
-        d1[1] = h.y;
-        d1[2] = h.z;
-      }
-      // every direction's loads are issued before any is used
-      float d[kSubMax][3];
+    // every load of the node is issued before any value is used (fixed nodes
+    // load too and then take w = 0)
+    const double vn[3] = {vcur[3 * g], vcur[3 * g + 1], vcur[3 * g + 2]};
+    const float4 h1 = A.vring[(int64_t)q.head * A.N + g];
+    SubDirs sd;
+    sub_load(A, g, q, sd);
+    const bool fx = A.fixed[g] != 0;
+    // d_1 is used only with a history (nh >= 1); this keeps its load with the
+    // others rather than in a branch of its own
+    asm volatile("" ::"v"(h1.x), "v"(h1.y), "v"(h1.z));
+    const float d1[3] = {h1.x, h1.y, h1.z};
+    double wv[3];
+    const double c0 = cf[0];
 #pragma unroll
-      for (int j = 1; j < kSubMax; ++j)
-        if (j < k) sub_dir(A, g, q, j, d1, vn, d[j]);
-      const double c0 = cf[0];
+    for (int x = 0; x < 3; ++x) wv[x] = c0 * vn[x];
 #pragma unroll
-      for (int x = 0; x < 3; ++x) wv[x] = c0 * vn[x];
+    for (int j = 1; j < kSubMax; ++j) {
+      if (j < k) {
+        const double cj = cf[j];
 #pragma unroll
-      for (int j = 1; j < kSubMax; ++j) {
-        if (j < k) {
-          const double cj = cf[j];
-#pragma unroll
-          for (int x = 0; x < 3; ++x) wv[x] += cj * (double)d[j][x];
-        }
+        for (int x = 0; x < 3; ++x) wv[x] += cj * (double)sub_get(sd, q, j, x, d1, vn);
       }
     }
+#pragma unroll
+    for (int x = 0; x < 3; ++x) wv[x] = fx ? 0.0 : wv[x];
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
       sp[3 * l + x] = wv[x];
@@ -1647,15 +1699,22 @@ __global__ __launch_bounds__(BLOCK) void k_sub_start(StepArgs A) {
     Dt[2] = td.z;
   }
   const int64_t S = A.S;
-  for (int s = ti.slot_begin + (int)threadIdx.x; s < ti.slot_begin + ti.n_slots; s += BLOCK) {
-    SlotRaw<ST> raw;
-    load_raw<ST>(A, S, s, raw);
+  // the next slot's record is in flight while one is applied (a thread's
+  // last load repeats the tile's last slot)
+  const int se = ti.slot_begin + ti.n_slots;
+  int s = ti.slot_begin + (int)threadIdx.x;
+  SlotRaw<ST> raw;
+  if (s < se) load_raw<ST>(A, S, s, raw);
+  for (; s < se; s += BLOCK) {
+    SlotRaw<ST> next;
+    load_raw<ST>(A, S, min(s + BLOCK, se - 1), next);
     double bt[3][3], D[3];
     unpack_raw<double, ST>(raw, Dt, bt, D);
     if constexpr (kOrdered)
       slot_spmv_store<double, double>(raw.c, bt, D, sp, A.ordered_terms + 12 * (int64_t)s);
     else
       slot_spmv<double, double, double>(raw.c, bt, D, sp, sy, no);
+    raw = next;
   }
   __syncthreads();
   double rr = 0.0;
