@@ -283,15 +283,36 @@ __global__ __launch_bounds__(kElemBlock) void k_element(StepArgs A) {
   __syncthreads();
 
   const int64_t S = A.S;
-  for (int s = ti.slot_begin + threadIdx.x; s < ti.slot_begin + ti.n_slots; s += kElemBlock) {
-    const ushort4 c = A.conn[s];
+  // a slot's corners, rest gradients and D; the next slot's are in flight
+  // while one is used (a thread's last load repeats the tile's last slot)
+  struct SlotIn {
+    ushort4 c;
+    double b[3][3], D[3];
+  };
+  auto load_in = [&](int s, SlotIn &in) {
+    in.c = A.conn[s];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) in.b[k][d] = A.rest64[(3 * k + d) * S + s];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) in.D[k] = A.vd64[k * S + s];
+  };
+  const int se = ti.slot_begin + ti.n_slots;
+  SlotIn cur;
+  if (ti.slot_begin + (int)threadIdx.x < se) load_in(ti.slot_begin + threadIdx.x, cur);
+  for (int s = ti.slot_begin + threadIdx.x; s < se; s += kElemBlock) {
+    SlotIn nxt;
+    load_in(min(s + kElemBlock, se - 1), nxt);
+    const ushort4 c = cur.c;
     const int cn[4] = {c.x, c.y, c.z, c.w};
     double b[3][3];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) b[k][d] = A.rest64[(3 * k + d) * S + s];
-    const double D0 = A.vd64[s], D1 = A.vd64[S + s], D2 = A.vd64[2 * S + s];
+      for (int d = 0; d < 3; ++d) b[k][d] = cur.b[k][d];
+    const double D0 = cur.D[0], D1 = cur.D[1], D2 = cur.D[2];
+    cur = nxt;
 
     // F = I + sum_k (u_k - u_0) b_k^T  (displacement form: no cancellation
     // between absolute coordinates)
