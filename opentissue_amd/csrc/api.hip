@@ -184,7 +184,7 @@ constexpr int kCgTail = 8;    // iterations per graph near the expected stop
 // Batched meshes shrink the grid to the tiles of running bodies once per
 // chunk, and a body that stops inside a chunk keeps its workgroups in the
 // grid until the chunk ends (each issues its prologue loads, then exits):
-// shorter chunks there (the combined 2,048-body set: 154.8 -> 149.7 ms of CG
+// shorter chunks there (the combined 2,048-body set: 154.7 -> 150.2 ms of CG
 // per learned step with 8 instead of 32; a single 64^3 body: 1% slower with
 // 16, from the extra graph launches).
 constexpr int kCgChunkBatched = 8;
