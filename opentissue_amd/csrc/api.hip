@@ -60,6 +60,7 @@ struct otfem_solver {
   // device buffers
   double *x = nullptr, *x0 = nullptr, *v = nullptr, *fext = nullptr, *mass = nullptr;
   double *rhs = nullptr, *r = nullptr, *p = nullptr, *y = nullptr;
+  float *y32 = nullptr;  // [2][3N] q = A p in FP32 (bodies with a small residual, FP32 p)
   uint8_t *fixed = nullptr;
   TileInfo *tiles = nullptr;
   int32_t *halo = nullptr;
@@ -204,7 +205,7 @@ void drop_graphs(otfem_solver *h) {
 
 void free_device(otfem_solver *h) {
   dfree(h->x); dfree(h->x0); dfree(h->v); dfree(h->fext); dfree(h->mass);
-  dfree(h->rhs); dfree(h->r); dfree(h->p); dfree(h->y); dfree(h->fixed);
+  dfree(h->rhs); dfree(h->r); dfree(h->p); dfree(h->y); dfree(h->y32); dfree(h->fixed);
   dfree(h->tiles); dfree(h->halo); dfree(h->conn); dfree(h->slot_tet);
   dfree(h->rest64); dfree(h->vd64); dfree(h->bt4); dfree(h->bt8); dfree(h->tile_d); dfree(h->tile_kb);
   dfree(h->Re); dfree(h->plastic); dfree(h->plastic_next); dfree(h->plastic_param);
@@ -389,6 +390,7 @@ otfem_status ensure_device(otfem_solver *h) {
   HIPCHK(h, dalloc(&h->r, 6 * N));
   HIPCHK(h, dalloc(&h->p, 6 * N));
   HIPCHK(h, dalloc(&h->y, 6 * N));
+  HIPCHK(h, dalloc(&h->y32, 6 * N));
   HIPCHK(h, dalloc(&h->fixed, N));
   HIPCHK(h, dalloc(&h->tiles, nt));
   HIPCHK(h, dalloc(&h->halo, L.halo.size()));
@@ -748,7 +750,7 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
 StepArgs make_args(otfem_solver *h, double dt, int warp, const otfem_params &P) {
   StepArgs a{};
   a.x = h->x; a.x0 = h->x0; a.v = h->v; a.vw = h->v; a.fext = h->fext; a.mass = h->mass; a.fixed = h->fixed;
-  a.rhs = h->rhs; a.r = h->r; a.p = h->p; a.y = h->y;
+  a.rhs = h->rhs; a.r = h->r; a.p = h->p; a.y = h->y; a.y32 = h->y32;
   a.tiles = h->tiles; a.halo = h->halo; a.conn = h->conn; a.slot_tet = h->slot_tet;
   a.rest64 = h->rest64; a.vd64 = h->vd64; a.bt4 = h->bt4; a.bt8 = h->bt8;
   a.Re = h->Re; a.plastic = h->plastic; a.plastic_next = h->plastic_next;
@@ -783,6 +785,13 @@ StepArgs make_args(otfem_solver *h, double dt, int warp, const otfem_params &P) 
   // iterate does not depend on them beyond the tolerance, while the
   // fixed-count iterate of the reference mode would
   a.p32 = (P.tol_mode == 1 && h->lc.spmv_mode == SPMV_F64) ? 1 : 0;
+  // and q = A p in FP32 once a body's relative residual is below 1e4 tol:
+  // r -= alpha q then drifts from the true residual by ~6e-8 of the later
+  // |alpha q|, far below tol (the FP64 oracle with q rounded this way: true
+  // relative residuals at the stop as with FP64 q; rounded from the first
+  // iteration, up to 9e-6 at tol 1e-6)
+  const double q32_rel = 1e4 * P.tol;
+  a.q32_rr = a.p32 ? q32_rel * q32_rel : 0.0;
   a.ordered = h->ordered ? 1 : 0;
   a.ord_begin = h->ord_begin;
   a.ord_ent = h->ord_ent;

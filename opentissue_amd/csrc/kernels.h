@@ -19,6 +19,7 @@ struct BodyState {
   int32_t stopped;  // 0 while running; else 1 + index of the CG kernel that recorded the stop
   int32_t converged;
   int32_t flags;  // bit 0: non-finite scalar
+  int32_t q32_from;  // first CG kernel of the step that stored q in FP32 (INT32_MAX until one does)
   double alpha_hist[2];  // alpha of iteration j-1, written by (even) CG kernel j at [j & 1]
 };
 
@@ -39,6 +40,7 @@ struct StepArgs {
   double *p;             // [2][3N] search direction (FP64), or [2][3N] FP32 in its first half
                          //         with `p32`
   double *y;             // [2][3N] q = A p = cM m p + cK K' p (0 at fixed nodes)
+  float *y32;            // [2][3N] q in FP32, for bodies whose residual is small (with p32)
   // tiles / slots
   const TileInfo *tiles;
   const int32_t *halo;
@@ -92,6 +94,10 @@ struct StepArgs {
   int32_t p32;           // the CG direction p is stored in FP32 (CG to a relative residual in
                          // the default SpMV mode): the SpMV, the dot products and the updates
                          // of v and r all use the stored value, so r stays the residual of v
+  double q32_rr;         // with p32: a body's q = A p is stored in FP32 (y32) from the first CG
+                         // kernel of the step at which r.r <= q32_rr b.b (0: never).  r -= alpha q
+                         // then drifts from the true residual by the rounding of alpha q, which
+                         // stays far below the stopping tolerance once |r| is that small
   // Ordered sums (deterministic mode): slots store their per-corner terms in
   // `ordered_terms` (slot-major: one contiguous record per slot) and every
   // owned node adds its terms in slot order, so results do not depend on

@@ -263,6 +263,7 @@ __global__ __launch_bounds__(kElemBlock) void k_element(StepArgs A) {
     w->stopped = 0;
     w->converged = 0;
     w->flags = 0;
+    w->q32_from = INT32_MAX;
   }
   if (tile == 0 && threadIdx.x == 0) *A.done_count = 0;
 
@@ -842,7 +843,10 @@ struct NodeIn {
   int32_t g;  // internal node id
   int32_t fx;
   double m;  // owned nodes only
-  double r[3], y[3], v[3];  // y: q = A p as the previous kernel stored it
+  double r[3], v[3];
+  // q = A p as the previous kernel stored it: FP64 bits, or (q_in32) the
+  // bits of an FP32 value in the low half (one register pair either way)
+  uint64_t yb[3];
   PT p[3];
   PT pp[3];  // p of two iterations back (odd kernels from 3 on, owned nodes)
 };
@@ -856,8 +860,7 @@ struct NodeIn {
 template <int BLOCK, int NB, typename PT, bool kDefer>
 __device__ __forceinline__ void load_nodes(const StepArgs &A, const TileInfo &ti, int it,
                                            int base, const double *Rp, const PT *Pp,
-                                           const double *Yp, const PT *Pq,
-                                           NodeIn<PT> (&nd)[NB]) {
+                                           const PT *Pq, NodeIn<PT> (&nd)[NB]) {
   const int nn = ti.n_owned + ti.n_halo, no = ti.n_owned;
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
@@ -882,10 +885,7 @@ __device__ __forceinline__ void load_nodes(const StepArgs &A, const TileInfo &ti
     for (int d = 0; d < 3; ++d) nd[k].r[d] = R[3 * g + d];
     if (it > 0) {
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        nd[k].p[d] = Pp[3 * g + d];
-        nd[k].y[d] = Yp[3 * g + d];
-      }
+      for (int d = 0; d < 3; ++d) nd[k].p[d] = Pp[3 * g + d];
       if (!kDefer || (it & 1)) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) nd[k].v[d] = l < no ? A.v[3 * g + d] : 0.0;
@@ -895,6 +895,29 @@ __device__ __forceinline__ void load_nodes(const StepArgs &A, const TileInfo &ti
         for (int d = 0; d < 3; ++d) nd[k].pp[d] = l < no ? Pq[3 * g + d] : (PT)0;
       }
     }
+  }
+}
+
+// Loads q = A p of the previous kernel for the nodes load_nodes set up, from
+// the FP32 copy when that kernel stored it so (q_in32: the same for the whole
+// workgroup; the first batch issues this after the other loads, once the
+// body's record says which copy it is).
+template <int NB, typename PT>
+__device__ __forceinline__ void load_y(int it, const double *Yp, const float *Yfp, bool q_in32,
+                                       NodeIn<PT> (&nd)[NB]) {
+  if (it == 0) return;
+  if (q_in32) {
+#pragma unroll
+    for (int k = 0; k < NB; ++k)
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        nd[k].yb[d] = (uint64_t)__float_as_uint(Yfp[3 * (int64_t)nd[k].g + d]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < NB; ++k)
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        nd[k].yb[d] = (uint64_t)__double_as_longlong(Yp[3 * (int64_t)nd[k].g + d]);
   }
 }
 
@@ -911,8 +934,8 @@ __device__ __forceinline__ void load_nodes(const StepArgs &A, const TileInfo &ti
 // body's stop.
 template <typename AT, typename PT, bool kDefer>
 __device__ __forceinline__ void update_node(const StepArgs &A, int it, bool stop, double alpha,
-                                            double alpha_prev,
-                                            double beta, int l, int no, const NodeIn<PT> &n,
+                                            double alpha_prev, double beta, bool q_in32, int l,
+                                            int no, const NodeIn<PT> &n,
                                             double *Rn, PT *Pn, PT *sp, double *sr,
                                             AT *sy, double *sm, double &pmax) {
   const int64_t g = n.g;
@@ -926,7 +949,9 @@ __device__ __forceinline__ void update_node(const StepArgs &A, int it, bool stop
   } else {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      rn[d] = n.r[d] - alpha * n.y[d];  // y holds q = A p of the previous kernel
+      const double q = q_in32 ? (double)__uint_as_float((uint32_t)n.yb[d])
+                              : __longlong_as_double((long long)n.yb[d]);
+      rn[d] = n.r[d] - alpha * q;
       pn[d] = rn[d] + beta * n.p[d];
     }
     if (l < no) {
@@ -968,18 +993,26 @@ __device__ __forceinline__ void update_node(const StepArgs &A, int it, bool stop
 }
 
 // Owned node l's contribution to the tile partials p.q, r.q, q.q, r.r, and
-// its q = A p = cM m p + cK K'p to memory (the next kernel's r -= alpha q
-// uses the same value).
+// its q = A p = cM m p + cK K'p to memory, in FP64 or (q_out32) rounded to
+// FP32 first: the partials and the next kernel's r -= alpha q use the same
+// value, so the recurrence for r.r and the update of r agree.
 template <typename AT, typename PS>
 __device__ __forceinline__ void node_partials(const StepArgs &A, int64_t g, int l, double m,
                                               const PS *sp, const double *sr, const AT *sy,
-                                              double *Yn, double4 &acc, double inv) {
+                                              double *Yn, float *Yfn, bool q_out32,
+                                              double4 &acc, double inv) {
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const double yd = acc_value<AT>(sy[3 * l + d], inv);
     const double p = sp[3 * l + d], r = sr[3 * l + d];
-    const double q = A.cM * m * p + A.cK * yd;
-    Yn[3 * g + d] = q;
+    double q = A.cM * m * p + A.cK * yd;
+    if (q_out32) {
+      const float qf = (float)q;
+      q = (double)qf;
+      Yfn[3 * g + d] = qf;
+    } else {
+      Yn[3 * g + d] = q;
+    }
     acc.x += p * q;
     acc.y += r * q;
     acc.z += q * q;
@@ -1036,14 +1069,21 @@ void k_cg_iter(StepArgs A, int it) {
   // batch of node inputs (b.b is read after the sums: loading it early
   // measured slower).
   const int st = *(volatile int32_t *)&bs->stopped;
+  // with an FP32 direction, q = A p goes to FP32 once the body's residual is
+  // small (StepArgs::q32_rr); q32_from is the kernel that first stored it so,
+  // and this kernel's workgroups may see it being set to `it` (not <= it - 1)
+  constexpr bool kQ32 = std::is_same<PT, float>::value;
+  const int q32_from = kQ32 ? bs->q32_from : INT32_MAX;
   if (!CgShape<BLOCK>::kEarlyPartials && st != 0 && st - 1 < it) return;
   const int t0 = ti.body_tile_begin, t1 = ti.body_tile_end;
   const int nt = A.ntiles;
   const int nn = ti.n_owned + ti.n_halo, no = ti.n_owned;
   const int cur = it & 1, prv = (it - 1) & 1;
   double *Rn = A.r + cur * 3 * A.N, *Yn = A.y + cur * 3 * A.N;
+  float *Yfn = A.y32 + cur * 3 * A.N;
   PT *Pn = reinterpret_cast<PT *>(A.p) + cur * 3 * A.N;
   const double *Rp = A.r + prv * 3 * A.N, *Yp = A.y + prv * 3 * A.N;
+  const float *Yfp = A.y32 + prv * 3 * A.N;
   const PT *Pp = reinterpret_cast<const PT *>(A.p) + prv * 3 * A.N;
   // alpha of iteration it-2, whose update of v the previous (even) kernel
   // deferred to this one
@@ -1070,7 +1110,9 @@ void k_cg_iter(StepArgs A, int it) {
       load_raw<ST>(A, S, min(sb + (int)threadIdx.x + u * BLOCK, send - 1), ra[u]);
   }
   NodeIn<PT> nd[NB];
-  load_nodes<BLOCK, NB, PT, kDefer>(A, ti, it, 0, Rp, Pp, Yp, Pn, nd);
+  load_nodes<BLOCK, NB, PT, kDefer>(A, ti, it, 0, Rp, Pp, Pn, nd);
+  const bool q_in32 = kQ32 && __builtin_amdgcn_readfirstlane(q32_from) <= it - 1;  // uniform
+  load_y<NB, PT>(it, Yp, Yfp, q_in32, nd);
 
   // exit if the body stopped in an earlier kernel (a stop recorded by this
   // kernel still needs this workgroup's prologue: v += alpha p)
@@ -1124,6 +1166,8 @@ void k_cg_iter(StepArgs A, int it) {
   }
   bool conv, nonfin;
   const bool stop = stop_rule(A, it, rr, bb, conv, nonfin);
+  // every workgroup of the body derives the same rr and bb, so the same choice
+  const bool q_out32 = kQ32 && (q_in32 || rr <= A.q32_rr * bb);
   if (ti.body_rank == 0 && threadIdx.x == 0) {
     if (it == 0) {
       bs->bb = bb;
@@ -1133,6 +1177,7 @@ void k_cg_iter(StepArgs A, int it) {
       record_stop(A, bs, it, rr, conv, nonfin, it);
     else
       bs->iterations = it;
+    if (q_out32 && !q_in32) bs->q32_from = it;
   }
 
   double *sr = lds;                 // r of owned nodes
@@ -1149,12 +1194,13 @@ void k_cg_iter(StepArgs A, int it) {
     for (int k = 0; k < NB; ++k) {
       const int l = base + (int)threadIdx.x + k * BLOCK;
       if (l < nn)
-        update_node<Acc, PT, kDefer>(A, it, stop, alpha, alpha_prev, beta, l, no, nd[k], Rn, Pn, sp, sr,
-                             sy, sm, pmax);
+        update_node<Acc, PT, kDefer>(A, it, stop, alpha, alpha_prev, beta, q_in32, l, no, nd[k],
+                                     Rn, Pn, sp, sr, sy, sm, pmax);
     }
     base += NB * BLOCK;
     if (base >= nn) break;
-    load_nodes<BLOCK, NB, PT, kDefer>(A, ti, it, base, Rp, Pp, Yp, Pn, nd);
+    load_nodes<BLOCK, NB, PT, kDefer>(A, ti, it, base, Rp, Pp, Pn, nd);
+    load_y<NB, PT>(it, Yp, Yfp, q_in32, nd);
   }
   if (stop) return;
   if (kFixed && pmax > 0.0) atomicMax(&s_pmax, (unsigned long long)__double_as_longlong(pmax));
@@ -1233,7 +1279,8 @@ void k_cg_iter(StepArgs A, int it) {
       sy[3 * l + 1] = y1;
       sy[3 * l + 2] = y2;
     }
-    if (m >= 0.0) node_partials<Acc, PT>(A, ti.node_begin + l, l, m, sp, sr, sy, Yn, acc, inv);
+    if (m >= 0.0)
+      node_partials<Acc, PT>(A, ti.node_begin + l, l, m, sp, sr, sy, Yn, Yfn, q_out32, acc, inv);
   }
   const double4 tot = block_sum4<BLOCK, false>(acc, red);
   if (threadIdx.x == 0) reinterpret_cast<double4 *>(A.part4 + cur * 4 * (int64_t)nt)[tile] = tot;
