@@ -193,11 +193,10 @@ otfem_status otfem_unconverged_totals(otfem_handle h, int64_t *totals, int reset
  * arithmetic on FP64-stored element data), and whether steps are replayed as
  * captured hipGraphs (1) or launched kernel by kernel (0).  In the default
  * mode with tol_mode 1 the CG direction p is stored in FP32 (rounded before
- * every use, so r stays the residual of v), and so is A p once a body's
- * relative residual is below 1e4 tol (rounded before the dot products and
- * r -= alpha A p; the drift this leaves between r and b - A v stays ~1% of
- * tol); v, r and every sum stay FP64.  No reference counterpart (tuning of
- * this implementation). */
+ * every use, so r stays the residual of v), and so are r and A p once a
+ * body's relative residual is below 1e4 tol (each rounded before any use;
+ * the drift this leaves between r and b - A v stays ~1% of tol); v and every
+ * sum stay FP64.  No reference counterpart (tuning of this implementation). */
 enum { OTFEM_SPMV_F32 = 0, OTFEM_SPMV_F32_ACC64 = 1, OTFEM_SPMV_F64 = 2, OTFEM_SPMV_F64_STORE64 = 3 };
 otfem_status otfem_set_kernel_config(otfem_handle h, int cg_block, int spmv_mode, int use_graphs);
 /* Deterministic mode (on = 1): every per-node sum of the step has a result

@@ -60,7 +60,7 @@ struct otfem_solver {
   // device buffers
   double *x = nullptr, *x0 = nullptr, *v = nullptr, *fext = nullptr, *mass = nullptr;
   double *rhs = nullptr, *r = nullptr, *p = nullptr, *y = nullptr;
-  float *y32 = nullptr;  // [2][3N] q = A p in FP32 (bodies with a small residual, FP32 p)
+  float *y32 = nullptr, *r32 = nullptr;  // [2][3N] q = A p and r in FP32 (small residual, FP32 p)
   uint8_t *fixed = nullptr;
   TileInfo *tiles = nullptr;
   int32_t *halo = nullptr;
@@ -205,7 +205,7 @@ void drop_graphs(otfem_solver *h) {
 
 void free_device(otfem_solver *h) {
   dfree(h->x); dfree(h->x0); dfree(h->v); dfree(h->fext); dfree(h->mass);
-  dfree(h->rhs); dfree(h->r); dfree(h->p); dfree(h->y); dfree(h->y32); dfree(h->fixed);
+  dfree(h->rhs); dfree(h->r); dfree(h->p); dfree(h->y); dfree(h->y32); dfree(h->r32); dfree(h->fixed);
   dfree(h->tiles); dfree(h->halo); dfree(h->conn); dfree(h->slot_tet);
   dfree(h->rest64); dfree(h->vd64); dfree(h->bt4); dfree(h->bt8); dfree(h->tile_d); dfree(h->tile_kb);
   dfree(h->Re); dfree(h->plastic); dfree(h->plastic_next); dfree(h->plastic_param);
@@ -391,6 +391,7 @@ otfem_status ensure_device(otfem_solver *h) {
   HIPCHK(h, dalloc(&h->p, 6 * N));
   HIPCHK(h, dalloc(&h->y, 6 * N));
   HIPCHK(h, dalloc(&h->y32, 6 * N));
+  HIPCHK(h, dalloc(&h->r32, 6 * N));
   HIPCHK(h, dalloc(&h->fixed, N));
   HIPCHK(h, dalloc(&h->tiles, nt));
   HIPCHK(h, dalloc(&h->halo, L.halo.size()));
@@ -750,7 +751,7 @@ otfem_status init_common(otfem_solver *h, const double *E, const double *nu, con
 StepArgs make_args(otfem_solver *h, double dt, int warp, const otfem_params &P) {
   StepArgs a{};
   a.x = h->x; a.x0 = h->x0; a.v = h->v; a.vw = h->v; a.fext = h->fext; a.mass = h->mass; a.fixed = h->fixed;
-  a.rhs = h->rhs; a.r = h->r; a.p = h->p; a.y = h->y; a.y32 = h->y32;
+  a.rhs = h->rhs; a.r = h->r; a.p = h->p; a.y = h->y; a.y32 = h->y32; a.r32 = h->r32;
   a.tiles = h->tiles; a.halo = h->halo; a.conn = h->conn; a.slot_tet = h->slot_tet;
   a.rest64 = h->rest64; a.vd64 = h->vd64; a.bt4 = h->bt4; a.bt8 = h->bt8;
   a.Re = h->Re; a.plastic = h->plastic; a.plastic_next = h->plastic_next;

@@ -41,6 +41,7 @@ struct StepArgs {
                          //         with `p32`
   double *y;             // [2][3N] q = A p = cM m p + cK K' p (0 at fixed nodes)
   float *y32;            // [2][3N] q in FP32, for bodies whose residual is small (with p32)
+  float *r32;            // [2][3N] r in FP32, likewise (a body switches both at once)
   // tiles / slots
   const TileInfo *tiles;
   const int32_t *halo;
@@ -94,10 +95,10 @@ struct StepArgs {
   int32_t p32;           // the CG direction p is stored in FP32 (CG to a relative residual in
                          // the default SpMV mode): the SpMV, the dot products and the updates
                          // of v and r all use the stored value, so r stays the residual of v
-  double q32_rr;         // with p32: a body's q = A p is stored in FP32 (y32) from the first CG
-                         // kernel of the step at which r.r <= q32_rr b.b (0: never).  r -= alpha q
-                         // then drifts from the true residual by the rounding of alpha q, which
-                         // stays far below the stopping tolerance once |r| is that small
+  double q32_rr;         // with p32: a body's q = A p and r are stored in FP32 (y32, r32) from
+                         // the first CG kernel of the step at which r.r <= q32_rr b.b (0: never).
+                         // r then drifts from the true residual by the rounding of alpha q and
+                         // of r, which stays far below the stopping tolerance once |r| is small
   // Ordered sums (deterministic mode): slots store their per-corner terms in
   // `ordered_terms` (slot-major: one contiguous record per slot) and every
   // owned node adds its terms in slot order, so results do not depend on

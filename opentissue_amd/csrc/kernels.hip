@@ -843,10 +843,10 @@ struct NodeIn {
   int32_t g;  // internal node id
   int32_t fx;
   double m;  // owned nodes only
-  double r[3], v[3];
-  // q = A p as the previous kernel stored it: FP64 bits, or (q_in32) the
-  // bits of an FP32 value in the low half (one register pair either way)
-  uint64_t yb[3];
+  double v[3];
+  // r and q = A p as the previous kernel stored them: FP64 bits, or (q_in32)
+  // the bits of an FP32 value in the low half (one register pair either way)
+  uint64_t rb[3], yb[3];
   PT p[3];
   PT pp[3];  // p of two iterations back (odd kernels from 3 on, owned nodes)
 };
@@ -859,8 +859,8 @@ struct NodeIn {
 // kernels only: even kernels defer their update of v to the next kernel.
 template <int BLOCK, int NB, typename PT, bool kDefer>
 __device__ __forceinline__ void load_nodes(const StepArgs &A, const TileInfo &ti, int it,
-                                           int base, const double *Rp, const PT *Pp,
-                                           const PT *Pq, NodeIn<PT> (&nd)[NB]) {
+                                           int base, const PT *Pp, const PT *Pq,
+                                           NodeIn<PT> (&nd)[NB]) {
   const int nn = ti.n_owned + ti.n_halo, no = ti.n_owned;
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
@@ -872,7 +872,6 @@ __device__ __forceinline__ void load_nodes(const StepArgs &A, const TileInfo &ti
       g = __ldg(&A.halo[ti.halo_begin + l - no]);
     nd[k].g = g;
   }
-  const double *R = it == 0 ? A.r : Rp;
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
     const int64_t g = nd[k].g;
@@ -881,8 +880,6 @@ __device__ __forceinline__ void load_nodes(const StepArgs &A, const TileInfo &ti
     // the mass enters q only, which the previous kernel stored: owned nodes
     // need it for this kernel's q, halo nodes not at all
     nd[k].m = l < no ? A.mass[g] : 0.0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) nd[k].r[d] = R[3 * g + d];
     if (it > 0) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) nd[k].p[d] = Pp[3 * g + d];
@@ -898,26 +895,34 @@ __device__ __forceinline__ void load_nodes(const StepArgs &A, const TileInfo &ti
   }
 }
 
-// Loads q = A p of the previous kernel for the nodes load_nodes set up, from
-// the FP32 copy when that kernel stored it so (q_in32: the same for the whole
-// workgroup; the first batch issues this after the other loads, once the
-// body's record says which copy it is).
+// Loads r and q = A p of the previous kernel for the nodes load_nodes set up,
+// from the FP32 copies when that kernel stored them so (q_in32: the same for
+// the whole workgroup; the first batch issues these after the other loads,
+// once the body's record says which copies they are).  Kernel 0 reads the
+// start's residual.
 template <int NB, typename PT>
-__device__ __forceinline__ void load_y(int it, const double *Yp, const float *Yfp, bool q_in32,
-                                       NodeIn<PT> (&nd)[NB]) {
-  if (it == 0) return;
-  if (q_in32) {
+__device__ __forceinline__ void load_rq(const StepArgs &A, int it, const double *Rp,
+                                        const float *Rfp, const double *Yp, const float *Yfp,
+                                        bool q_in32, NodeIn<PT> (&nd)[NB]) {
+  if (q_in32) {  // it >= 1
 #pragma unroll
     for (int k = 0; k < NB; ++k)
 #pragma unroll
-      for (int d = 0; d < 3; ++d)
-        nd[k].yb[d] = (uint64_t)__float_as_uint(Yfp[3 * (int64_t)nd[k].g + d]);
+      for (int d = 0; d < 3; ++d) {
+        const int64_t i = 3 * (int64_t)nd[k].g + d;
+        nd[k].rb[d] = (uint64_t)__float_as_uint(Rfp[i]);
+        nd[k].yb[d] = (uint64_t)__float_as_uint(Yfp[i]);
+      }
   } else {
+    const double *R = it == 0 ? A.r : Rp;
 #pragma unroll
     for (int k = 0; k < NB; ++k)
 #pragma unroll
-      for (int d = 0; d < 3; ++d)
-        nd[k].yb[d] = (uint64_t)__double_as_longlong(Yp[3 * (int64_t)nd[k].g + d]);
+      for (int d = 0; d < 3; ++d) {
+        const int64_t i = 3 * (int64_t)nd[k].g + d;
+        nd[k].rb[d] = (uint64_t)__double_as_longlong(R[i]);
+        if (it > 0) nd[k].yb[d] = (uint64_t)__double_as_longlong(Yp[i]);
+      }
   }
 }
 
@@ -934,27 +939,33 @@ __device__ __forceinline__ void load_y(int it, const double *Yp, const float *Yf
 // body's stop.
 template <typename AT, typename PT, bool kDefer>
 __device__ __forceinline__ void update_node(const StepArgs &A, int it, bool stop, double alpha,
-                                            double alpha_prev, double beta, bool q_in32, int l,
-                                            int no, const NodeIn<PT> &n,
-                                            double *Rn, PT *Pn, PT *sp, double *sr,
+                                            double alpha_prev, double beta, bool q_in32,
+                                            bool q_out32, int l, int no, const NodeIn<PT> &n,
+                                            double *Rn, float *Rfn, PT *Pn, PT *sp, double *sr,
                                             AT *sy, double *sm, double &pmax) {
   const int64_t g = n.g;
   double rn[3], pn[3];
   if (n.fx) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) rn[d] = pn[d] = 0.0;
-  } else if (it == 0) {
-#pragma unroll
-    for (int d = 0; d < 3; ++d) rn[d] = pn[d] = n.r[d];
   } else {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const double q = q_in32 ? (double)__uint_as_float((uint32_t)n.yb[d])
-                              : __longlong_as_double((long long)n.yb[d]);
-      rn[d] = n.r[d] - alpha * q;
-      pn[d] = rn[d] + beta * n.p[d];
+      const double r = q_in32 ? (double)__uint_as_float((uint32_t)n.rb[d])
+                              : __longlong_as_double((long long)n.rb[d]);
+      if (it == 0) {
+        rn[d] = r;
+      } else {
+        const double q = q_in32 ? (double)__uint_as_float((uint32_t)n.yb[d])
+                                : __longlong_as_double((long long)n.yb[d]);
+        rn[d] = r - alpha * q;  // q = A p of the previous kernel
+      }
+      // once the body's residual is small r is kept in FP32 like q, rounded
+      // before any use
+      if (q_out32) rn[d] = (double)(float)rn[d];
+      pn[d] = it == 0 ? rn[d] : rn[d] + beta * n.p[d];
     }
-    if (l < no) {
+    if (it > 0 && l < no) {
       if (!kDefer || (it & 1)) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -981,9 +992,17 @@ __device__ __forceinline__ void update_node(const StepArgs &A, int it, bool stop
     pmax = isfinite(pn[d]) ? fmax(pmax, fabs(pn[d])) : HUGE_VAL;
   }
   if (l < no) {
+    // kernel 0 leaves the start's residual where it is, unless it already
+    // goes to the FP32 copy
+    const bool store_r = !stop && (it > 0 || q_out32);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      if (it > 0 && !stop) Rn[3 * g + d] = rn[d];
+      if (store_r) {
+        if (q_out32)
+          Rfn[3 * g + d] = (float)rn[d];
+        else
+          Rn[3 * g + d] = rn[d];
+      }
       Pn[3 * g + d] = pf[d];
       sr[3 * l + d] = rn[d];
       sy[3 * l + d] = (AT)0;
@@ -1080,10 +1099,10 @@ void k_cg_iter(StepArgs A, int it) {
   const int nn = ti.n_owned + ti.n_halo, no = ti.n_owned;
   const int cur = it & 1, prv = (it - 1) & 1;
   double *Rn = A.r + cur * 3 * A.N, *Yn = A.y + cur * 3 * A.N;
-  float *Yfn = A.y32 + cur * 3 * A.N;
+  float *Rfn = A.r32 + cur * 3 * A.N, *Yfn = A.y32 + cur * 3 * A.N;
   PT *Pn = reinterpret_cast<PT *>(A.p) + cur * 3 * A.N;
   const double *Rp = A.r + prv * 3 * A.N, *Yp = A.y + prv * 3 * A.N;
-  const float *Yfp = A.y32 + prv * 3 * A.N;
+  const float *Rfp = A.r32 + prv * 3 * A.N, *Yfp = A.y32 + prv * 3 * A.N;
   const PT *Pp = reinterpret_cast<const PT *>(A.p) + prv * 3 * A.N;
   // alpha of iteration it-2, whose update of v the previous (even) kernel
   // deferred to this one
@@ -1110,9 +1129,9 @@ void k_cg_iter(StepArgs A, int it) {
       load_raw<ST>(A, S, min(sb + (int)threadIdx.x + u * BLOCK, send - 1), ra[u]);
   }
   NodeIn<PT> nd[NB];
-  load_nodes<BLOCK, NB, PT, kDefer>(A, ti, it, 0, Rp, Pp, Pn, nd);
+  load_nodes<BLOCK, NB, PT, kDefer>(A, ti, it, 0, Pp, Pn, nd);
   const bool q_in32 = kQ32 && __builtin_amdgcn_readfirstlane(q32_from) <= it - 1;  // uniform
-  load_y<NB, PT>(it, Yp, Yfp, q_in32, nd);
+  load_rq<NB, PT>(A, it, Rp, Rfp, Yp, Yfp, q_in32, nd);
 
   // exit if the body stopped in an earlier kernel (a stop recorded by this
   // kernel still needs this workgroup's prologue: v += alpha p)
@@ -1194,13 +1213,13 @@ void k_cg_iter(StepArgs A, int it) {
     for (int k = 0; k < NB; ++k) {
       const int l = base + (int)threadIdx.x + k * BLOCK;
       if (l < nn)
-        update_node<Acc, PT, kDefer>(A, it, stop, alpha, alpha_prev, beta, q_in32, l, no, nd[k],
-                                     Rn, Pn, sp, sr, sy, sm, pmax);
+        update_node<Acc, PT, kDefer>(A, it, stop, alpha, alpha_prev, beta, q_in32, q_out32, l,
+                                     no, nd[k], Rn, Rfn, Pn, sp, sr, sy, sm, pmax);
     }
     base += NB * BLOCK;
     if (base >= nn) break;
-    load_nodes<BLOCK, NB, PT, kDefer>(A, ti, it, base, Rp, Pp, Pn, nd);
-    load_y<NB, PT>(it, Yp, Yfp, q_in32, nd);
+    load_nodes<BLOCK, NB, PT, kDefer>(A, ti, it, base, Pp, Pn, nd);
+    load_rq<NB, PT>(A, it, Rp, Rfp, Yp, Yfp, q_in32, nd);
   }
   if (stop) return;
   if (kFixed && pmax > 0.0) atomicMax(&s_pmax, (unsigned long long)__double_as_longlong(pmax));
