@@ -64,7 +64,8 @@ struct otfem_solver {
   uint8_t *fixed = nullptr;
   TileInfo *tiles = nullptr;
   int32_t *halo = nullptr;
-  ushort4 *conn = nullptr;
+  uint32_t *conn_lo = nullptr;  // slot corners, 12 bits each (see StepArgs)
+  uint16_t *conn_hi = nullptr;
   int32_t *slot_tet = nullptr;
   double *rest64 = nullptr, *vd64 = nullptr;
   float4 *bt4 = nullptr;
@@ -206,7 +207,7 @@ void drop_graphs(otfem_solver *h) {
 void free_device(otfem_solver *h) {
   dfree(h->x); dfree(h->x0); dfree(h->v); dfree(h->fext); dfree(h->mass);
   dfree(h->rhs); dfree(h->r); dfree(h->p); dfree(h->y); dfree(h->y32); dfree(h->r32); dfree(h->fixed);
-  dfree(h->tiles); dfree(h->halo); dfree(h->conn); dfree(h->slot_tet);
+  dfree(h->tiles); dfree(h->halo); dfree(h->conn_lo); dfree(h->conn_hi); dfree(h->slot_tet);
   dfree(h->rest64); dfree(h->vd64); dfree(h->bt4); dfree(h->bt8); dfree(h->tile_d); dfree(h->tile_kb);
   dfree(h->Re); dfree(h->plastic); dfree(h->plastic_next); dfree(h->plastic_param);
   dfree(h->body); dfree(h->part_bb); dfree(h->part_rr);
@@ -395,7 +396,8 @@ otfem_status ensure_device(otfem_solver *h) {
   HIPCHK(h, dalloc(&h->fixed, N));
   HIPCHK(h, dalloc(&h->tiles, nt));
   HIPCHK(h, dalloc(&h->halo, L.halo.size()));
-  HIPCHK(h, dalloc(&h->conn, S));
+  HIPCHK(h, dalloc(&h->conn_lo, S));
+  HIPCHK(h, dalloc(&h->conn_hi, S));
   HIPCHK(h, dalloc(&h->slot_tet, S));
   HIPCHK(h, dalloc(&h->rest64, 9 * S));
   HIPCHK(h, dalloc(&h->vd64, 3 * S));
@@ -422,8 +424,19 @@ otfem_status ensure_device(otfem_solver *h) {
   if (!L.halo.empty())
     HIPCHK(h, hipMemcpyAsync(h->halo, L.halo.data(), L.halo.size() * sizeof(int32_t),
                              hipMemcpyHostToDevice, s));
+  std::vector<uint32_t> conn_lo;  // packed corners, alive until the sync below
+  std::vector<uint16_t> conn_hi;
   if (S > 0) {  // a mesh may have no tets (every node fixed)
-    HIPCHK(h, hipMemcpyAsync(h->conn, L.slot_conn.data(), S * sizeof(ushort4), hipMemcpyHostToDevice, s));
+    conn_lo.resize(S);
+    conn_hi.resize(S);
+    for (int64_t k = 0; k < S; ++k) {
+      const uint32_t c0 = L.slot_conn[4 * k], c1 = L.slot_conn[4 * k + 1];
+      const uint32_t c2 = L.slot_conn[4 * k + 2], c3 = L.slot_conn[4 * k + 3];
+      conn_lo[k] = c0 | (c1 << 12) | ((c2 & 0xffu) << 24);
+      conn_hi[k] = (uint16_t)((c2 >> 8) | (c3 << 4));
+    }
+    HIPCHK(h, hipMemcpyAsync(h->conn_lo, conn_lo.data(), S * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIPCHK(h, hipMemcpyAsync(h->conn_hi, conn_hi.data(), S * sizeof(uint16_t), hipMemcpyHostToDevice, s));
     HIPCHK(h, hipMemcpyAsync(h->slot_tet, L.slot_tet.data(), S * sizeof(int32_t), hipMemcpyHostToDevice, s));
   }
   std::vector<double> tmp;
@@ -752,7 +765,7 @@ StepArgs make_args(otfem_solver *h, double dt, int warp, const otfem_params &P) 
   StepArgs a{};
   a.x = h->x; a.x0 = h->x0; a.v = h->v; a.vw = h->v; a.fext = h->fext; a.mass = h->mass; a.fixed = h->fixed;
   a.rhs = h->rhs; a.r = h->r; a.p = h->p; a.y = h->y; a.y32 = h->y32; a.r32 = h->r32;
-  a.tiles = h->tiles; a.halo = h->halo; a.conn = h->conn; a.slot_tet = h->slot_tet;
+  a.tiles = h->tiles; a.halo = h->halo; a.conn_lo = h->conn_lo; a.conn_hi = h->conn_hi; a.slot_tet = h->slot_tet;
   a.rest64 = h->rest64; a.vd64 = h->vd64; a.bt4 = h->bt4; a.bt8 = h->bt8;
   a.Re = h->Re; a.plastic = h->plastic; a.plastic_next = h->plastic_next;
   a.plastic_param = h->plastic_param;
@@ -1019,7 +1032,7 @@ otfem_status otfem_create(const otfem_mesh_desc *d, otfem_handle *out) {
     if (B > 1 && (!d->body_node_offsets || !d->body_tet_offsets))
       return bad("num_bodies > 1 needs body_node_offsets and body_tet_offsets");
     std::string e = build_layout(N, T, h->x0_host.data(), h->tets.data(), B, d->body_node_offsets,
-                                 d->body_tet_offsets, d->target_tile_nodes, 16384, h->L);
+                                 d->body_tet_offsets, d->target_tile_nodes, kMaxTileNodes, h->L);
     if (!e.empty()) return bad(e.c_str());
     *out = h.release();
     return OTFEM_OK;

@@ -45,7 +45,11 @@ struct StepArgs {
   // tiles / slots
   const TileInfo *tiles;
   const int32_t *halo;
-  const ushort4 *conn;
+  // slot corners: four tile-local node indices of 12 bits each (tile tables
+  // hold at most kMaxTileNodes nodes), c0 | c1 << 12 | (c2 & 0xff) << 24 in
+  // conn_lo and c2 >> 8 | c3 << 4 in conn_hi: 6 bytes a slot
+  const uint32_t *conn_lo;
+  const uint16_t *conn_hi;
   const int32_t *slot_tet;
   const double *rest64;  // [9][S]  grad N_k (k = 1..3) at rest
   const double *vd64;    // [3][S]  |V| * (D0, D1, D2)
@@ -130,6 +134,8 @@ struct StepArgs {
 // the upper triangle of D^T A D (kSubPairs, entry (i, j), i <= j, at
 // j (j + 1) / 2 + i) followed by D^T r_plain (kSubMax).
 constexpr int kSubMax = 12;
+// Largest tile node table (owned + halo) the 12-bit slot corners address.
+constexpr int kMaxTileNodes = 4096;
 constexpr int kSubPairs = kSubMax * (kSubMax + 1) / 2;
 constexpr int kSubPart = kSubPairs + kSubMax;
 

@@ -244,6 +244,17 @@ __device__ __forceinline__ void matvec(const T *M, const T *x, T *y) {
 
 }  // namespace
 
+// The four tile-local corners of slot s (12 bits each, packed in 6 bytes).
+__device__ __forceinline__ ushort4 slot_corners(const StepArgs &A, int s) {
+  const uint32_t lo = A.conn_lo[s], hi = A.conn_hi[s];
+  ushort4 c;
+  c.x = (unsigned short)(lo & 0xfffu);
+  c.y = (unsigned short)((lo >> 12) & 0xfffu);
+  c.z = (unsigned short)((lo >> 24) | ((hi & 0xfu) << 8));
+  c.w = (unsigned short)(hi >> 4);
+  return c;
+}
+
 __global__ __launch_bounds__(kElemBlock) void k_element(StepArgs A) {
   extern __shared__ double lds[];
   __shared__ double red[kElemBlock / 64 + 1];
@@ -291,7 +302,7 @@ __global__ __launch_bounds__(kElemBlock) void k_element(StepArgs A) {
     double b[3][3], D[3];
   };
   auto load_in = [&](int s, SlotIn &in) {
-    in.c = A.conn[s];
+    in.c = slot_corners(A, s);
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
@@ -643,21 +654,21 @@ __device__ __forceinline__ void load_raw(const StepArgs &A, int64_t S, int s, Sl
 template <>
 __device__ __forceinline__ void load_raw<float>(const StepArgs &A, int64_t S, int s,
                                                 SlotRaw<float> &r) {
-  r.c = A.conn[s];
+  r.c = slot_corners(A, s);
 #pragma unroll
   for (int k = 0; k < 3; ++k) r.q[k] = A.bt4[k * S + s];
 }
 template <>
 __device__ __forceinline__ void load_raw<double>(const StepArgs &A, int64_t S, int s,
                                                  SlotRaw<double> &r) {
-  r.c = A.conn[s];
+  r.c = slot_corners(A, s);
 #pragma unroll
   for (int k = 0; k < 3; ++k) r.q[k] = A.bt8[k * S + s];
 }
 template <>
 __device__ __forceinline__ void load_raw<Folded>(const StepArgs &A, int64_t S, int s,
                                                  SlotRaw<Folded> &r) {
-  r.c = A.conn[s];
+  r.c = slot_corners(A, s);
   r.q[0] = A.bt4[s];
   r.q[1] = A.bt4[S + s];
   r.z = reinterpret_cast<const float *>(A.bt4 + 2 * S)[s];
