@@ -466,11 +466,14 @@ def test_stop_measures_the_true_residual(mode):
     * FP64 element records (SPMV_F64_STORE64): the GPU's operator is the
       oracle's to rounding, and |b - A v_gpu| / |b| <= 1e-6 (measured equal
       to the GPU's own recursive residual to 4 digits);
-    * the default (SPMV_F64: FP32 slot records, FP32 direction p):
+    * the default (SPMV_F64: FP32 slot records, FP32 direction p, and r and
+      A p in FP32 once the relative residual is below 1e-2):
       |b - A v_gpu| / |b| <= 2.5e-6.  The excess over 1e-6 is the FP32
       rounding of the element records, whose node sums cancel (measured
-      0.97-2.1e-6, the same with FP64 directions), not the direction's
-      storage: p is rounded before every use, so r stays the residual of v.
+      0.97-2.1e-6, the same with FP64 directions), not the vectors'
+      storage: p is rounded before every use, so r stays the residual of v,
+      and the rounding of r and A p below the switch moves r from b - A v by
+      about 1% of the tolerance.
     The displacement agrees with the oracle's converged step to 1e-6."""
     from opentissue_amd import fem, mesh
     import oracle
